@@ -1,0 +1,165 @@
+/*
+ * mfgpu: MI355X-native matrix-free operators and MG-preconditioned CG.
+ *
+ * C ABI of libmfgpu.so.  Every entry point returns an int status (MFG_OK = 0)
+ * and never throws; mfg_last_error() returns the message of the last failure
+ * on the calling thread.  Vector arguments are caller-owned DEVICE pointers
+ * in the operator's precision; the library owns device copies of index maps,
+ * constraint lists, geometry and quadrature caches.  All work of one object
+ * is ordered on that object's HIP stream (default: the null stream), which
+ * makes an object thread-compatible but not thread-safe.
+ *
+ * Reference interfaces replaced (davidscn/matrix-free-dealii-precice):
+ *   NeoHookOperator<dim,Number>    include/solid_mechanics/mf_nh_operator.h:99-237
+ *   LaplaceOperator<dim,number>    include/heat_transfer/heat_transfer.h:81-127
+ *   MatrixFree data layout build   include/solid_mechanics/mf_elasticity.h:725-1055
+ *   MGTransferMatrixFree / PreconditionChebyshev / SolverCG / Multigrid as
+ *   configured at mf_elasticity.h:1140-1279, 1981-2070, heat_transfer.h:757-812
+ */
+#ifndef MFGPU_H
+#define MFGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- status --- */
+#define MFG_OK 0
+#define MFG_ERR_ARGUMENT 1   /* invalid argument or unsupported configuration */
+#define MFG_ERR_HIP 2        /* a HIP runtime call failed */
+#define MFG_ERR_NUMERIC 3    /* det F <= 0, non-positive diagonal, breakdown */
+#define MFG_ERR_NOCONV 4     /* iteration cap reached (SolverControl::NoConvergence) */
+#define MFG_ERR_STATE 5      /* called before the required set-up step */
+
+#define MFG_FP64 0
+#define MFG_FP32 1
+
+/* MF caching strategies (parameter_handling.h "MF caching"); the device path
+ * implements tensor2, the strategy whose data the canonical roofline counts. */
+#define MFG_CACHING_SCALAR_REFERENTIAL 0
+#define MFG_CACHING_TENSOR2 1
+#define MFG_CACHING_TENSOR4 2
+#define MFG_CACHING_TENSOR4_NS 3
+
+/* Last error message of the calling thread ("" if none). */
+const char *mfg_last_error(void);
+/* Library version string. */
+const char *mfg_version(void);
+
+/* ------------------------------------------------- mesh / level layout --- */
+/* Host-side description of one mesh level: the data a MatrixFree object is
+ * built from (mf_elasticity.h:933-992, 996-1055).
+ *   cell_nodes:       n_cells*(degree+1)^3 node indices, lexicographic within
+ *                     the cell (a + N*(b + N*c)); dof = n_components*node+comp
+ *   cell_vertices:    n_cells*8*3 vertex coordinates, vertex v = vx+2vy+4vz;
+ *                     geometry is the trilinear map (MappingQ1)
+ *   constrained_dofs: sorted homogeneous Dirichlet dofs (identity rows)
+ * Node indices must be < 2^28.  n_q_1d must equal degree+1. */
+typedef struct mfg_level_desc {
+  int32_t degree;
+  int32_t n_q_1d;
+  int32_t n_components;
+  int32_t n_cells;
+  int64_t n_nodes;
+  const int32_t *cell_nodes;
+  const double *cell_vertices;
+  const int32_t *constrained_dofs;
+  int64_t n_constrained;
+} mfg_level_desc;
+
+/* Structured box mesh builder (replaces GridGenerator::subdivided_hyper_rectangle
+ * + DoFHandler::distribute_dofs + make_constraints, mf_elasticity.h:659-721,
+ * 1918-1974, for global refinement of a box).  face_mask bit f constrains
+ * boundary face f in the colorize numbering (0:x=lo,1:x=hi,2:y=lo,3:y=hi,
+ * 4:z=lo,5:z=hi); comp_mask selects the constrained components. */
+typedef struct mfg_mesh_t *mfg_mesh;
+int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n_cells[3], int32_t degree,
+                        int32_t n_components, int32_t face_mask, int32_t comp_mask, mfg_mesh *out);
+int mfg_box_mesh_destroy(mfg_mesh mesh);
+/* Fills desc with pointers into the mesh's host arrays (valid while mesh lives). */
+int mfg_box_mesh_describe(mfg_mesh mesh, mfg_level_desc *desc);
+/* Node coordinates, n_nodes*3 doubles, host memory. */
+int mfg_box_mesh_node_coords(mfg_mesh mesh, double *out);
+
+/* ------------------------------------------------------------ operators --- */
+typedef struct mfg_operator_t *mfg_operator;
+
+/* Neo-Hookean material (material.h:78-94) and Newmark factor alpha_1 =
+ * 1/(beta dt^2) (mf_elasticity.h:240); formulation 0 or 1; caching one of
+ * MFG_CACHING_*.  Replaces NeoHookOperator::initialize + set_material
+ * (mf_nh_operator.h:115-128, 331-388). */
+typedef struct mfg_solid_params {
+  double mu, nu, rho, alpha_1;
+  int32_t formulation;
+  int32_t caching;
+} mfg_solid_params;
+
+int mfg_solid_create(const mfg_level_desc *level, const mfg_solid_params *params, int32_t precision,
+                     mfg_operator *out);
+/* LaplaceOperator (heat_transfer.h:81-250): A = M + dt*c(x)*K.  coefficient is
+ * a host table per (cell, q) or NULL for c == 1 (Coefficient, :52-78). */
+int mfg_heat_create(const mfg_level_desc *level, double dt, const double *coefficient, int32_t precision,
+                    mfg_operator *out);
+int mfg_operator_destroy(mfg_operator op);                         /* clear() :112-113 */
+int mfg_operator_set_stream(mfg_operator op, void *hip_stream);
+int64_t mfg_operator_m(mfg_operator op);                             /* m() :133 */
+int64_t mfg_operator_n(mfg_operator op);                             /* n() :135 */
+int32_t mfg_operator_precision(mfg_operator op);
+
+/* cache(): linearise at the device displacement u (precision u_precision),
+ * NeoHookOperator::cache (mf_nh_operator.h:392-523).  MFG_ERR_NUMERIC if
+ * det F <= 0 at any quadrature point. */
+int mfg_solid_cache(mfg_operator op, const void *u, int32_t u_precision);
+
+/* vmult / vmult_add / Tvmult / Tvmult_add (mf_nh_operator.h:543-625,
+ * MatrixFreeOperators::Base::vmult for heat): dst = A src (dst += A src),
+ * identity rows on constrained dofs.  Tvmult == vmult (A is symmetric). */
+int mfg_operator_vmult(mfg_operator op, void *dst, const void *src);
+int mfg_operator_vmult_add(mfg_operator op, void *dst, const void *src);
+int mfg_operator_Tvmult(mfg_operator op, void *dst, const void *src);
+int mfg_operator_Tvmult_add(mfg_operator op, void *dst, const void *src);
+
+/* compute_diagonal (mf_nh_operator.h:1023-1060, heat_transfer.h:205-230):
+ * diagonal with constrained entries set to 1, and its inverse. */
+int mfg_operator_compute_diagonal(mfg_operator op);
+/* Copies (device to device) the diagonal / inverse diagonal; either may be NULL.
+ * get_matrix_diagonal_inverse (:157-161). */
+int mfg_operator_copy_diagonal(mfg_operator op, void *diag, void *inv_diag);
+/* el(row, col): diagonal entry only (:1064-1073). */
+int mfg_operator_el(mfg_operator op, int64_t row, int64_t col, double *value);
+/* precondition_Jacobi: dst = omega * D^{-1} src (:281-291). */
+int mfg_operator_precondition_jacobi(mfg_operator op, void *dst, const void *src, double omega);
+/* memory_consumption (:241-260): bytes held for vmult (caches + layout). */
+size_t mfg_operator_memory_consumption(mfg_operator op);
+/* Quadrature-point cache as stored on the device (structure of arrays,
+ * field-major: field f of point (cell, q) at f*n_cells*nq3 + cell*nq3 + q).
+ * Solid tensor2 formulation 1: 18 fields  Jinv_cur[f][e] (9), JxW_cur,
+ * 2(mu-2 lambda ln J)/J, 1/J, tau/J (xx,yy,zz,xy,xz,yz).  Heat: 11 fields
+ * Jinv (9), JxW, c.  Copies n_fields*n_qp values to the device pointer out;
+ * n_fields may be queried with out == NULL. */
+int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields);
+
+/* Kernel timing (TimerOutput-style sections of the reference,
+ * mf_elasticity.h:427-432): when enabled, every vmult/vmult_add records HIP
+ * events on the operator's stream around its cell kernel.  mfg_operator_timing
+ * waits for the recorded events, returns the summed cell-kernel time and the
+ * launch count since the last query, and resets both. */
+int mfg_operator_enable_timing(mfg_operator op, int32_t enable);
+int mfg_operator_timing(mfg_operator op, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------------ synthetic inputs --- */
+/* dst[i] = 2*U(seed,i) - 1 with U = (splitmix64(seed,i) >> 11) * 2^-53, on the
+ * device (precision MFG_FP64/MFG_FP32), ordered on `stream`. */
+int mfg_fill_uniform(uint64_t seed, int64_t n, int32_t precision, void *dst, void *stream);
+/* Linearisation point on the host: nodal interpolant of
+ * amp*sin(pi x)sin(pi y)sin(pi z) per component plus noise*U[-1,1] (seeded as
+ * mfg_fill_uniform, indexed by dof), constrained dofs zero. */
+int mfg_box_mesh_fill_u0(mfg_mesh mesh, double amp, double noise, uint64_t seed, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFGPU_H */

@@ -1,0 +1,417 @@
+// Sum-factorisation cell kernels for gfx950 (CDNA4, wave64).
+//
+// Layout of one launch: a workgroup processes CPB cells, one thread per
+// quadrature point (n_q = p+1, so points and nodes share the N^3 index space
+// and the thread also owns node (i,j,k) of its cell).  The three 1D sweeps of
+// each evaluate/integrate pass go through an LDS tile per cell; the
+// collocation derivative at the Gauss points turns the three gradient
+// directions into one LDS round trip.  Gather is a coalesced read of the
+// int32 cell->node map followed by per-node vector loads; the scatter is a
+// hardware FP64/FP32 atomic add (constrained dofs are skipped via the
+// constraint bits stored above bit 28 of every map entry).
+//
+// Algorithmic reference: FEEvaluation::evaluate/integrate with
+// values|gradients as used by NeoHookOperator::do_operation_on_cell
+// (mf_nh_operator.h:651-1019) and LaplaceOperator::do_operation_on_cell
+// (heat_transfer.h:234-250).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mfg {
+
+constexpr int kConstraintShift = 28;
+constexpr uint32_t kNodeMaskDev = (1u << kConstraintShift) - 1u;
+
+template <int P, int CPB>
+constexpr int block_threads() {
+  return ((CPB * (P + 1) * (P + 1) * (P + 1) + 63) / 64) * 64;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD walks one contiguous range of cells (shared nodes stay in one L2).
+__device__ inline int xcd_remap(int b, int G) {
+  const int x = b & 7, l = b >> 3, q = G >> 3, r = G & 7;
+  return x * q + (x < r ? x : r) + l;
+}
+
+__device__ inline void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ inline void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// Per-thread view of the cell tile in LDS.
+template <int N, typename Number>
+struct Tile {
+  static constexpr int N3 = N * N * N;
+  Number *base;  // slot 0 of this cell
+  int i, j, k, pt;
+  bool writer;   // false for padding lanes
+  __device__ Number *slot(int s) const { return base + s * N3; }
+  __device__ void put(int s, Number v) const {
+    if (writer) base[s * N3 + pt] = v;
+  }
+  // sum_a M[row*N + a] * slot(a along dir)
+  template <int DIR>
+  __device__ Number line(const Number *M, int row, int s) const {
+    const Number *src = slot(s);
+    Number r = 0;
+    if (DIR == 0) {
+      const Number *b = src + N * (j + N * k);
+#pragma unroll
+      for (int a = 0; a < N; ++a) r += M[row * N + a] * b[a];
+    } else if (DIR == 1) {
+      const Number *b = src + i + N * N * k;
+#pragma unroll
+      for (int a = 0; a < N; ++a) r += M[row * N + a] * b[a * N];
+    } else {
+      const Number *b = src + i + N * j;
+#pragma unroll
+      for (int a = 0; a < N; ++a) r += M[row * N + a] * b[a * N * N];
+    }
+    return r;
+  }
+  // sum_a M[a*N + col] * slot(a along dir)   (transposed matrix)
+  template <int DIR>
+  __device__ Number line_t(const Number *M, int col, int s) const {
+    const Number *src = slot(s);
+    Number r = 0;
+    if (DIR == 0) {
+      const Number *b = src + N * (j + N * k);
+#pragma unroll
+      for (int a = 0; a < N; ++a) r += M[a * N + col] * b[a];
+    } else if (DIR == 1) {
+      const Number *b = src + i + N * N * k;
+#pragma unroll
+      for (int a = 0; a < N; ++a) r += M[a * N + col] * b[a * N];
+    } else {
+      const Number *b = src + i + N * j;
+#pragma unroll
+      for (int a = 0; a < N; ++a) r += M[a * N + col] * b[a * N * N];
+    }
+    return r;
+  }
+};
+
+// Evaluate values and reference-coordinate gradients at the quadrature points
+// of NC components; slots 0..2NC-1 are used.  On return the B slots hold the
+// point values (still read by later lines until the next barrier).
+template <int N, int NC, typename Number>
+__device__ inline void evaluate_vg(const Tile<N, Number> &T, const Number *S, const Number *D, const Number (&u)[NC],
+                                   Number (&val)[NC], Number (&grad)[NC][3]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(c, u[c]);
+  __syncthreads();
+  Number t[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) t[c] = T.template line<0>(S, T.i, c);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(NC + c, t[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) t[c] = T.template line<1>(S, T.j, NC + c);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(c, t[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) val[c] = T.template line<2>(S, T.k, c);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(NC + c, val[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    grad[c][0] = T.template line<0>(D, T.i, NC + c);
+    grad[c][1] = T.template line<1>(D, T.j, NC + c);
+    grad[c][2] = T.template line<2>(D, T.k, NC + c);
+  }
+}
+
+// Integrate submitted values and reference gradients back to the nodes
+// (transpose of evaluate_vg).  Uses all 3NC slots.
+template <int N, int NC, typename Number>
+__device__ inline void integrate_vg(const Tile<N, Number> &T, const Number *S, const Number *D,
+                                    const Number (&val)[NC], const Number (&grad)[NC][3], Number (&out)[NC]) {
+  __syncthreads();  // B slots may still be read by evaluate_vg's last lines
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) T.put(3 * c + f, grad[c][f]);
+  __syncthreads();
+  Number r[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    r[c] = val[c] + T.template line_t<0>(D, T.i, 3 * c + 0) + T.template line_t<1>(D, T.j, 3 * c + 1) +
+           T.template line_t<2>(D, T.k, 3 * c + 2);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(c, r[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) r[c] = T.template line_t<2>(S, T.k, c);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(NC + c, r[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) r[c] = T.template line_t<1>(S, T.j, NC + c);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) T.put(c, r[c]);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = T.template line_t<0>(S, T.i, c);
+}
+
+// Evaluate only reference gradients (cache()); same slot usage as evaluate_vg.
+template <int N, int NC, typename Number>
+__device__ inline void evaluate_g(const Tile<N, Number> &T, const Number *S, const Number *D, const Number (&u)[NC],
+                                  Number (&grad)[NC][3]) {
+  Number val[NC];
+  evaluate_vg<N, NC, Number>(T, S, D, u, val, grad);
+}
+
+// ------------------------------------------------------ quadrature ops ---
+// Each op provides load(qidx) -> Data and apply(Data, val, grad), turning
+// values/reference gradients at a point into submitted value and reference
+// gradient already multiplied by JxW.
+
+// Solid, tensor2 caching, formulation 1 (mf_nh_operator.h:945-974).
+// Fields: Jinv_cur[f][e] (9), JxW, c2 = 2(mu-2 lambda lnJ)/J, c1 = 1/J,
+// T = tau/J (xx,yy,zz,xy,xz,yz).
+template <typename Number>
+struct QSolidT2F1 {
+  static constexpr int NC = 3;
+  static constexpr int kFields = 18;
+  const Number *qd;
+  int64_t nqp;
+  Number two_lambda, rho_alpha;
+  struct Data {
+    Number ji[9], jxw, c2, c1, T[6];
+  };
+  __device__ Data load(int64_t q) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ji[f] = qd[f * nqp + q];
+    d.jxw = qd[9 * nqp + q];
+    d.c2 = qd[10 * nqp + q];
+    d.c1 = qd[11 * nqp + q];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) d.T[f] = qd[(12 + f) * nqp + q];
+    return d;
+  }
+  __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
+    Number g[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) g[c][e] = grad[c][0] * d.ji[0 * 3 + e] + grad[c][1] * d.ji[1 * 3 + e] + grad[c][2] * d.ji[2 * 3 + e];
+    const Number tr = g[0][0] + g[1][1] + g[2][2];
+    const Number diag = two_lambda * d.c1 * tr;
+    const Number half_c2 = Number(0.5) * d.c2;
+    // T symmetric: T[0]=xx T[1]=yy T[2]=zz T[3]=xy T[4]=xz T[5]=yz
+    const Number Tm[3][3] = {{d.T[0], d.T[3], d.T[4]}, {d.T[3], d.T[1], d.T[5]}, {d.T[4], d.T[5], d.T[2]}};
+    Number Pm[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        Pm[a][b] = half_c2 * (g[a][b] + g[b][a]) + (a == b ? diag : Number(0)) + g[a][0] * Tm[0][b] +
+                   g[a][1] * Tm[1][b] + g[a][2] * Tm[2][b];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        grad[c][f] = d.jxw * (Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2]);
+      val[c] = d.jxw * rho_alpha * d.c1 * val[c];
+    }
+  }
+};
+
+// Solid, tensor2 caching, formulation 0 (mf_nh_operator.h:701-840).
+// Fields: Jinv_cur (9), JxW, J^{-1/3}, F (9).
+template <typename Number>
+struct QSolidT2F0 {
+  static constexpr int NC = 3;
+  static constexpr int kFields = 20;
+  const Number *qd;
+  int64_t nqp;
+  Number kappa, c_1, rho_alpha;
+  struct Data {
+    Number ji[9], jxw, jm13, F[9];
+  };
+  __device__ Data load(int64_t q) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ji[f] = qd[f * nqp + q];
+    d.jxw = qd[9 * nqp + q];
+    d.jm13 = qd[10 * nqp + q];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.F[f] = qd[(11 + f) * nqp + q];
+    return d;
+  }
+  __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
+    Number g[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) g[c][e] = grad[c][0] * d.ji[0 * 3 + e] + grad[c][1] * d.ji[1 * 3 + e] + grad[c][2] * d.ji[2 * 3 + e];
+    const Number *F = d.F;
+    const Number det_F = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                         F[2] * (F[3] * F[7] - F[4] * F[6]);
+    Number Fb[9];
+#pragma unroll
+    for (int a = 0; a < 9; ++a) Fb[a] = F[a] * d.jm13;
+    Number tau_bar[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        tau_bar[a][b] = Number(2) * c_1 * (Fb[a * 3 + 0] * Fb[b * 3 + 0] + Fb[a * 3 + 1] * Fb[b * 3 + 1] + Fb[a * 3 + 2] * Fb[b * 3 + 2]);
+    const Number tr_tau_bar = tau_bar[0][0] + tau_bar[1][1] + tau_bar[2][2];
+    const Number tr_tau_bar_dim = tr_tau_bar / Number(3);
+    const Number inv_det = Number(1) / det_F;
+    const Number dPsi = (kappa / Number(2)) * (det_F - inv_det);
+    const Number dPsi_J = dPsi * det_F;
+    const Number d2Psi = (kappa / Number(2)) * (Number(1) + inv_det * inv_det);
+    Number s[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) s[a][b] = Number(0.5) * (g[a][b] + g[b][a]);
+    const Number tr = s[0][0] + s[1][1] + s[2][2];
+    Number tau_iso_src = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) tau_iso_src += (tau_bar[a][b] - (a == b ? tr_tau_bar_dim : Number(0))) * s[a][b];
+    const Number vol = det_F * (dPsi + det_F * d2Psi) * tr;
+    Number Pm[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const Number tau_iso = tau_bar[a][b] - (a == b ? tr_tau_bar_dim : Number(0));
+        const Number dev = s[a][b] - (a == b ? tr / Number(3) : Number(0));
+        Number jc = s[a][b] * (-dPsi_J * Number(2)) + Number(2.0 / 3.0) * tr_tau_bar * dev -
+                    Number(2.0 / 3.0) * tr * tau_iso;
+        if (a == b) jc += vol - Number(2.0 / 3.0) * tau_iso_src;
+        Number geo = 0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const Number tau_mb = tau_bar[m][b] + (m == b ? dPsi_J - tr_tau_bar_dim : Number(0));
+          geo += g[a][m] * tau_mb;
+        }
+        Pm[a][b] = (jc + geo) * inv_det;
+      }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        grad[c][f] = d.jxw * (Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2]);
+      val[c] = d.jxw * rho_alpha * inv_det * val[c];
+    }
+  }
+};
+
+// Heat: value v, gradient c*dt*grad v (heat_transfer.h:242-248).
+// Fields: Jinv (9), JxW, c.
+template <typename Number>
+struct QHeat {
+  static constexpr int NC = 1;
+  static constexpr int kFields = 11;
+  const Number *qd;
+  int64_t nqp;
+  Number dt;
+  struct Data {
+    Number ji[9], jxw, c;
+  };
+  __device__ Data load(int64_t q) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ji[f] = qd[f * nqp + q];
+    d.jxw = qd[9 * nqp + q];
+    d.c = qd[10 * nqp + q];
+    return d;
+  }
+  __device__ void apply(const Data &d, Number (&val)[1], Number (&grad)[1][3]) const {
+    Number gx[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) gx[e] = grad[0][0] * d.ji[e] + grad[0][1] * d.ji[3 + e] + grad[0][2] * d.ji[6 + e];
+    const Number s = d.c * dt * d.jxw;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) grad[0][f] = s * (gx[0] * d.ji[f * 3] + gx[1] * d.ji[f * 3 + 1] + gx[2] * d.ji[f * 3 + 2]);
+    val[0] = d.jxw * val[0];
+  }
+};
+
+// ------------------------------------------------------ apply kernel ---
+// fix_mode: 0 no constrained-row handling, 1 dst[c] = src[c], 2 dst[c] += src[c]
+template <typename Number>
+struct ApplyArgs {
+  const uint32_t *cell_nodes;
+  int n_cells;
+  const Number *src;
+  Number *dst;
+  const Number *shape;   // N x N
+  const Number *colloc;  // N x N
+  const int32_t *constrained;
+  int64_t n_constrained;
+  int fix_mode;
+};
+
+template <int P, int CPB, typename Number, class QOp>
+__global__ void __launch_bounds__((block_threads<P, CPB>()))
+    apply_cells(ApplyArgs<Number> a, QOp qop) {
+  constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC, NT = block_threads<P, CPB>();
+  __shared__ Number s_shape[N * N], s_colloc[N * N];
+  __shared__ Number s_tile[CPB * 3 * NC * N3];
+  const int t = threadIdx.x;
+  if (t < N * N) {
+    s_shape[t] = a.shape[t];
+    s_colloc[t] = a.colloc[t];
+  }
+  // rows of constrained dofs: identity (mf_nh_operator.h:622-624)
+  if (a.fix_mode) {
+    const int64_t g = (int64_t)blockIdx.x * NT + t;
+    for (int64_t idx = g; idx < a.n_constrained; idx += (int64_t)gridDim.x * NT) {
+      const int64_t dof = a.constrained[idx];
+      if (a.fix_mode == 1) a.dst[dof] = a.src[dof];
+      else a.dst[dof] += a.src[dof];
+    }
+  }
+  const int lc_raw = t / N3;
+  const bool pad = lc_raw >= CPB;
+  const int lc = pad ? CPB - 1 : lc_raw;
+  const int pt = pad ? 0 : t - lc_raw * N3;
+  Tile<N, Number> T;
+  T.base = s_tile + lc * 3 * NC * N3;
+  T.pt = pt;
+  T.i = pt % N;
+  T.j = (pt / N) % N;
+  T.k = pt / (N * N);
+  T.writer = !pad;
+  const int cell = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool active = !pad && cell < a.n_cells;
+  const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
+
+  uint32_t entry = active ? a.cell_nodes[qidx] : 0u;
+  const int64_t node = entry & kNodeMaskDev;
+  Number u[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const bool skip = !active || ((entry >> (kConstraintShift + c)) & 1u);
+    u[c] = skip ? Number(0) : a.src[node * NC + c];
+  }
+  const typename QOp::Data qd = qop.load(qidx);
+  __syncthreads();
+
+  Number val[NC], grad[NC][3];
+  evaluate_vg<N, NC, Number>(T, s_shape, s_colloc, u, val, grad);
+  qop.apply(qd, val, grad);
+  Number out[NC];
+  integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (!((entry >> (kConstraintShift + c)) & 1u)) atomic_add(&a.dst[node * NC + c], out[c]);
+  }
+}
+
+}  // namespace mfg

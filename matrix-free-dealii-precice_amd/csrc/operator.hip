@@ -1,0 +1,531 @@
+// Device operators behind the C ABI: the neo-Hookean tangent operator
+// (NeoHookOperator, mf_nh_operator.h) and the heat operator (LaplaceOperator,
+// heat_transfer.h:81-250).  Owns the device copy of the level layout, the
+// quadrature cache and the diagonal; vectors are caller-owned device memory.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "operator.hpp"
+#include "setup_kernels.hpp"
+#include "vector_kernels.hpp"
+
+using namespace mfg;
+
+namespace mfg {
+
+// cells per workgroup of the apply / cache kernels, per degree
+template <int P>
+struct DefaultCPB {
+  static constexpr int value = P == 1 ? 8 : P == 2 ? 7 : P == 6 ? 1 : 2;
+};
+
+template <typename Number>
+static void fill(Number *x, int64_t n, Number v, hipStream_t s) {
+  if (n <= 0) return;
+  const int threads = 256;
+  const int blocks = (int)std::min<int64_t>((n + threads - 1) / threads, 65536);
+  fill_kernel<Number><<<blocks, threads, 0, s>>>(x, n, v);
+  MFG_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------- dispatchers ---
+template <int P, typename Number>
+static void launch_apply(const OperatorData &op, const void *src, void *dst, int fix_mode) {
+  constexpr int CPB = DefaultCPB<P>::value;
+  ApplyArgs<Number> a;
+  a.cell_nodes = op.cell_nodes.as<uint32_t>();
+  a.n_cells = op.n_cells;
+  a.src = (const Number *)src;
+  a.dst = (Number *)dst;
+  a.shape = op.tab_shape.as<Number>();
+  a.colloc = op.tab_colloc.as<Number>();
+  a.constrained = op.constrained.as<int32_t>();
+  a.n_constrained = op.n_constrained;
+  a.fix_mode = fix_mode;
+  const int grid = blocks_for(op.n_cells, CPB);
+  constexpr int NT = block_threads<P, CPB>();
+  if (op.kind == 0) {
+    if (op.sp.formulation == 1) {
+      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+      apply_cells<P, CPB, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+    } else {
+      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
+      apply_cells<P, CPB, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+    }
+  } else {
+    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+    apply_cells<P, CPB, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+  }
+  MFG_HIP(hipGetLastError());
+}
+
+template <int P, typename Number>
+static void launch_cache(OperatorData &op, const void *u, int u_fp32) {
+  constexpr int CPB = DefaultCPB<P>::value;
+  CacheArgs<Number> a;
+  a.cell_nodes = op.cell_nodes.as<uint32_t>();
+  a.n_cells = op.n_cells;
+  a.u = u;
+  a.u_fp32 = u_fp32;
+  a.vertices = op.vertices.as<double>();
+  a.shape = op.tab_shape.as<Number>();
+  a.colloc = op.tab_colloc.as<Number>();
+  a.points = op.tab_points.as<double>();
+  a.weights = op.tab_weights.as<double>();
+  a.qd = op.qd.as<Number>();
+  a.nqp = op.nqp;
+  a.mu = Number(op.sp.mu);
+  a.lambda = Number(op.lambda);
+  a.formulation = op.sp.formulation;
+  a.error = op.flag.as<int>();
+  const int grid = blocks_for(op.n_cells, CPB);
+  cache_solid<P, CPB, Number><<<grid, block_threads<P, CPB>(), 0, op.stream>>>(a);
+  MFG_HIP(hipGetLastError());
+}
+
+template <int P, typename Number>
+static void launch_diagonal(OperatorData &op) {
+  DiagArgs<Number> a;
+  a.cell_nodes = op.cell_nodes.as<uint32_t>();
+  a.n_cells = op.n_cells;
+  a.sq = op.tab_sq.as<Number>();
+  a.diag = op.diag.as<Number>();
+  constexpr int NT = ((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64;
+  if (op.kind == 0) {
+    if (op.sp.formulation == 1) {
+      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+      diagonal_cells<P, Number, QSolidT2F1<Number>><<<op.n_cells, NT, 0, op.stream>>>(a, q);
+    } else {
+      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
+      diagonal_cells<P, Number, QSolidT2F0<Number>><<<op.n_cells, NT, 0, op.stream>>>(a, q);
+    }
+  } else {
+    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+    diagonal_cells<P, Number, QHeat<Number>><<<op.n_cells, NT, 0, op.stream>>>(a, q);
+  }
+  MFG_HIP(hipGetLastError());
+}
+
+#define MFG_DISPATCH_DEGREE(p, FN, ...)                                                  \
+  switch (p) {                                                                           \
+    case 1: FN<1, Number>(__VA_ARGS__); break;                                           \
+    case 2: FN<2, Number>(__VA_ARGS__); break;                                           \
+    case 3: FN<3, Number>(__VA_ARGS__); break;                                           \
+    case 4: FN<4, Number>(__VA_ARGS__); break;                                           \
+    case 5: FN<5, Number>(__VA_ARGS__); break;                                           \
+    case 6: FN<6, Number>(__VA_ARGS__); break;                                           \
+    default: throw Error(MFG_ERR_ARGUMENT, "unsupported degree");                        \
+  }
+
+template <typename Number>
+static void apply_t(const OperatorData &op, const void *src, void *dst, int fix_mode) {
+  MFG_DISPATCH_DEGREE(op.p, launch_apply, op, src, dst, fix_mode);
+}
+template <typename Number>
+static void cache_t(OperatorData &op, const void *u, int u_fp32) {
+  MFG_DISPATCH_DEGREE(op.p, launch_cache, op, u, u_fp32);
+}
+template <typename Number>
+static void diagonal_t(OperatorData &op) {
+  MFG_DISPATCH_DEGREE(op.p, launch_diagonal, op);
+}
+
+void OperatorData::apply(const void *src, void *dst, bool add) const {
+  MFG_REQUIRE(ready, MFG_ERR_STATE, "operator used before cache()");
+  MFG_REQUIRE(src && dst, MFG_ERR_ARGUMENT, "null vector");
+  if (precision == MFG_FP64) {
+    if (!add) fill<double>((double *)dst, n_dofs, 0.0, stream);
+    timing_begin();
+    apply_t<double>(*this, src, dst, add ? 2 : 1);
+    timing_end();
+  } else {
+    if (!add) fill<float>((float *)dst, n_dofs, 0.0f, stream);
+    timing_begin();
+    apply_t<float>(*this, src, dst, add ? 2 : 1);
+    timing_end();
+  }
+}
+
+// Event pairs are reused; at most kMaxEvents launches are kept between
+// queries (older ones are folded in by an early collect).
+static constexpr size_t kMaxEvents = 2048;
+
+void OperatorData::timing_begin() const {
+  if (!timing) return;
+  if (events_used == events.size()) {
+    if (events.size() >= kMaxEvents) throw Error(MFG_ERR_STATE, "too many timed launches between queries");
+    hipEvent_t a, b;
+    MFG_HIP(hipEventCreate(&a));
+    MFG_HIP(hipEventCreate(&b));
+    events.emplace_back(a, b);
+  }
+  MFG_HIP(hipEventRecord(events[events_used].first, stream));
+}
+
+void OperatorData::timing_end() const {
+  if (!timing) return;
+  MFG_HIP(hipEventRecord(events[events_used].second, stream));
+  ++events_used;
+}
+
+void OperatorData::timing_collect(double &ms, int64_t &n) {
+  ms = 0;
+  n = (int64_t)events_used;
+  for (size_t i = 0; i < events_used; ++i) {
+    MFG_HIP(hipEventSynchronize(events[i].second));
+    float t = 0;
+    MFG_HIP(hipEventElapsedTime(&t, events[i].first, events[i].second));
+    ms += t;
+  }
+  events_used = 0;
+}
+
+OperatorData::~OperatorData() {
+  for (auto &e : events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+}
+
+void OperatorData::compute_diagonal() {
+  MFG_REQUIRE(ready, MFG_ERR_STATE, "compute_diagonal before cache()");
+  const size_t es = precision == MFG_FP64 ? 8 : 4;
+  if (diag.bytes != n_dofs * es) diag.allocate(n_dofs * es);
+  if (inv_diag.bytes != n_dofs * es) inv_diag.allocate(n_dofs * es);
+  fill<int>(flag.as<int>(), 1, 0, stream);
+  const int threads = 256;
+  if (precision == MFG_FP64) {
+    fill<double>(diag.as<double>(), n_dofs, 0.0, stream);
+    diagonal_t<double>(*this);
+    if (n_constrained)
+      set_entries_kernel<double><<<blocks_for(n_constrained, threads), threads, 0, stream>>>(
+          diag.as<double>(), constrained.as<int32_t>(), n_constrained, 1.0);
+    invert_kernel<double><<<std::min(blocks_for(n_dofs, threads), 65536), threads, 0, stream>>>(
+        diag.as<double>(), inv_diag.as<double>(), n_dofs, flag.as<int>());
+  } else {
+    fill<float>(diag.as<float>(), n_dofs, 0.0f, stream);
+    diagonal_t<float>(*this);
+    if (n_constrained)
+      set_entries_kernel<float><<<blocks_for(n_constrained, threads), threads, 0, stream>>>(
+          diag.as<float>(), constrained.as<int32_t>(), n_constrained, 1.0f);
+    invert_kernel<float><<<std::min(blocks_for(n_dofs, threads), 65536), threads, 0, stream>>>(
+        diag.as<float>(), inv_diag.as<float>(), n_dofs, flag.as<int>());
+  }
+  MFG_HIP(hipGetLastError());
+  int h = 0;
+  MFG_HIP(hipMemcpyAsync(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+  MFG_HIP(hipStreamSynchronize(stream));
+  MFG_REQUIRE(h == 0, MFG_ERR_NUMERIC, "No diagonal entry in a positive definite operator should be zero or negative");
+  has_diagonal = true;
+}
+
+void OperatorData::cache(const void *u, int u_precision) {
+  MFG_REQUIRE(kind == 0, MFG_ERR_ARGUMENT, "cache() applies to the solid operator");
+  MFG_REQUIRE(u, MFG_ERR_ARGUMENT, "null displacement");
+  fill<int>(flag.as<int>(), 1, 0, stream);
+  if (precision == MFG_FP64) cache_t<double>(*this, u, u_precision == MFG_FP32);
+  else cache_t<float>(*this, u, u_precision == MFG_FP32);
+  int h = 0;
+  MFG_HIP(hipMemcpyAsync(&h, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+  MFG_HIP(hipStreamSynchronize(stream));
+  MFG_REQUIRE(h == 0, MFG_ERR_NUMERIC, "det_F is not positive");
+  ready = true;
+  has_diagonal = false;
+}
+
+template <typename Number>
+static void upload_tables(OperatorData &op, const Tables1D &t) {
+  const int N = t.N;
+  std::vector<Number> sh(N * N), co(N * N), sq(3 * N * N);
+  for (int x = 0; x < N * N; ++x) {
+    sh[x] = (Number)t.shape[x];
+    co[x] = (Number)t.colloc[x];
+  }
+  // derivative of the GLL basis at the Gauss points: colloc * shape
+  for (int q = 0; q < N; ++q)
+    for (int a = 0; a < N; ++a) {
+      double s = t.shape[q * N + a], d = 0;
+      for (int r = 0; r < N; ++r) d += t.colloc[q * N + r] * t.shape[r * N + a];
+      sq[0 * N * N + q * N + a] = (Number)(s * s);
+      sq[1 * N * N + q * N + a] = (Number)(s * d);
+      sq[2 * N * N + q * N + a] = (Number)(d * d);
+    }
+  upload(op.tab_shape, sh.data(), sh.size(), op.stream);
+  upload(op.tab_colloc, co.data(), co.size(), op.stream);
+  upload(op.tab_sq, sq.data(), sq.size(), op.stream);
+  upload(op.tab_points, t.points.data(), t.points.size(), op.stream);
+  upload(op.tab_weights, t.weight.data(), t.weight.size(), op.stream);
+}
+
+void OperatorData::setup_layout(const mfg_level_desc &d) {
+  MFG_REQUIRE(d.degree >= 1 && d.degree <= kMaxDegree, MFG_ERR_ARGUMENT, "degree must be in 1..6");
+  MFG_REQUIRE(d.n_q_1d == d.degree + 1, MFG_ERR_ARGUMENT, "n_q_1d must equal degree+1");
+  MFG_REQUIRE(d.n_components == (kind == 0 ? 3 : 1), MFG_ERR_ARGUMENT, "wrong number of components");
+  MFG_REQUIRE(d.n_cells > 0 && d.cell_nodes && d.cell_vertices, MFG_ERR_ARGUMENT, "empty level");
+  MFG_REQUIRE(d.n_nodes > 0 && d.n_nodes <= (int64_t)kNodeMask, MFG_ERR_ARGUMENT, "node count out of range");
+  MFG_REQUIRE(d.n_constrained == 0 || d.constrained_dofs, MFG_ERR_ARGUMENT, "null constraint list");
+  p = d.degree;
+  nc = d.n_components;
+  n_cells = d.n_cells;
+  n_nodes = d.n_nodes;
+  n_dofs = n_nodes * nc;
+  const int N3 = (p + 1) * (p + 1) * (p + 1);
+  nqp = (int64_t)n_cells * N3;
+  n_constrained = d.n_constrained;
+  // constraint bits per node (component c -> bit 28+c)
+  std::vector<uint8_t> cmask(n_nodes, 0);
+  for (int64_t x = 0; x < d.n_constrained; ++x) {
+    const int64_t dof = d.constrained_dofs[x];
+    MFG_REQUIRE(dof >= 0 && dof < n_dofs, MFG_ERR_ARGUMENT, "constrained dof out of range");
+    cmask[dof / nc] |= (uint8_t)(1u << (dof % nc));
+  }
+  std::vector<uint32_t> map((size_t)nqp);
+  for (int64_t x = 0; x < nqp; ++x) {
+    const int32_t node = d.cell_nodes[x];
+    MFG_REQUIRE(node >= 0 && node < n_nodes, MFG_ERR_ARGUMENT, "cell node index out of range");
+    map[x] = (uint32_t)node | ((uint32_t)cmask[node] << kNodeBits);
+  }
+  upload(cell_nodes, map.data(), map.size(), stream);
+  upload(constrained, d.constrained_dofs, (size_t)n_constrained, stream);
+  host_constrained.assign(d.constrained_dofs, d.constrained_dofs + n_constrained);
+  upload(vertices, d.cell_vertices, (size_t)n_cells * 24, stream);
+  flag.allocate(sizeof(int));
+  const Tables1D t = make_tables(p);
+  if (precision == MFG_FP64) upload_tables<double>(*this, t);
+  else upload_tables<float>(*this, t);
+}
+
+template <typename Number>
+static void heat_geometry(OperatorData &op, const mfg_level_desc &d, const double *coef) {
+  // per (cell, q): Jinv (9), JxW, c -- MappingQ1 data (heat_transfer.h:483-496)
+  const Tables1D t = make_tables(op.p);
+  const int N = op.p + 1, N3 = N * N * N;
+  std::vector<Number> h((size_t)11 * op.nqp);
+  const int64_t n = op.nqp;
+  for (int c = 0; c < op.n_cells; ++c)
+    for (int q = 0; q < N3; ++q) {
+      const int i = q % N, j = (q / N) % N, k = q / (N * N);
+      const double x[3] = {t.points[i], t.points[j], t.points[k]};
+      double J[3][3] = {};
+      for (int v = 0; v < 8; ++v) {
+        const int b[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+        double nn[3], dd[3];
+        for (int e = 0; e < 3; ++e) {
+          nn[e] = b[e] ? x[e] : 1 - x[e];
+          dd[e] = b[e] ? 1 : -1;
+        }
+        const double g[3] = {dd[0] * nn[1] * nn[2], nn[0] * dd[1] * nn[2], nn[0] * nn[1] * dd[2]};
+        for (int e = 0; e < 3; ++e)
+          for (int f = 0; f < 3; ++f) J[e][f] += d.cell_vertices[(size_t)c * 24 + v * 3 + e] * g[f];
+      }
+      const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      MFG_REQUIRE(det > 0, MFG_ERR_ARGUMENT, "inverted cell");
+      const double id = 1.0 / det;
+      const double R[9] = {(J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                           (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id, (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id,
+                           (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id,
+                           (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                           (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id};
+      const int64_t o = (int64_t)c * N3 + q;
+      for (int f = 0; f < 9; ++f) h[f * n + o] = (Number)R[f];
+      h[9 * n + o] = (Number)(det * t.weight[i] * t.weight[j] * t.weight[k]);
+      h[10 * n + o] = (Number)(coef ? coef[o] : 1.0);
+    }
+  upload(op.qd, h.data(), h.size(), op.stream);
+}
+
+void OperatorData::setup_heat(const mfg_level_desc &d, const double *coef) {
+  if (precision == MFG_FP64) heat_geometry<double>(*this, d, coef);
+  else heat_geometry<float>(*this, d, coef);
+  n_fields = 11;
+  ready = true;
+}
+
+void OperatorData::setup_solid() {
+  // material.h:78-94
+  const double mu = sp.mu, nu = sp.nu;
+  kappa = (2.0 * mu * (1.0 + nu)) / (3.0 * (1.0 - 2.0 * nu));
+  c_1 = mu / 2.0;
+  lambda = (2.0 * mu * nu) / (1.0 - 2.0 * nu);
+  rho_alpha = sp.rho * sp.alpha_1;
+  n_fields = sp.formulation == 1 ? 18 : 20;
+  const size_t es = precision == MFG_FP64 ? 8 : 4;
+  qd.allocate((size_t)n_fields * nqp * es);
+}
+
+size_t OperatorData::memory_consumption() const {
+  return qd.bytes + cell_nodes.bytes + constrained.bytes;
+}
+
+}  // namespace mfg
+
+// --------------------------------------------------------------- C ABI ---
+extern "C" {
+
+int mfg_solid_create(const mfg_level_desc *level, const mfg_solid_params *params, int32_t precision,
+                     mfg_operator *out) {
+  return guarded([&] {
+    MFG_REQUIRE(level && params && out, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(precision == MFG_FP64 || precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
+    MFG_REQUIRE(params->formulation == 0 || params->formulation == 1, MFG_ERR_ARGUMENT,
+                "Unknown material formulation");
+    MFG_REQUIRE(params->caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
+                "the device operator implements the tensor2 caching");
+    MFG_REQUIRE(params->mu > 0 && params->nu > -1 && params->nu < 0.5, MFG_ERR_ARGUMENT, "bad material");
+    auto op = new mfg_operator_t();
+    try {
+      op->kind = 0;
+      op->precision = precision;
+      op->sp = *params;
+      op->setup_layout(*level);
+      op->setup_solid();
+    } catch (...) {
+      delete op;
+      throw;
+    }
+    *out = op;
+  });
+}
+
+int mfg_heat_create(const mfg_level_desc *level, double dt, const double *coefficient, int32_t precision,
+                    mfg_operator *out) {
+  return guarded([&] {
+    MFG_REQUIRE(level && out, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(dt > 0, MFG_ERR_ARGUMENT, "delta_t must be positive");
+    MFG_REQUIRE(precision == MFG_FP64 || precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
+    auto op = new mfg_operator_t();
+    try {
+      op->kind = 1;
+      op->precision = precision;
+      op->dt = dt;
+      op->setup_layout(*level);
+      op->setup_heat(*level, coefficient);
+    } catch (...) {
+      delete op;
+      throw;
+    }
+    *out = op;
+  });
+}
+
+int mfg_operator_destroy(mfg_operator op) {
+  return guarded([&] { delete op; });
+}
+
+int mfg_operator_set_stream(mfg_operator op, void *s) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    op->stream = (hipStream_t)s;
+  });
+}
+
+int64_t mfg_operator_m(mfg_operator op) { return op ? op->n_dofs : -1; }
+int64_t mfg_operator_n(mfg_operator op) { return op ? op->n_dofs : -1; }
+int32_t mfg_operator_precision(mfg_operator op) { return op ? op->precision : -1; }
+
+int mfg_solid_cache(mfg_operator op, const void *u, int32_t u_precision) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    op->cache(u, u_precision);
+  });
+}
+
+int mfg_operator_vmult(mfg_operator op, void *dst, const void *src) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    op->apply(src, dst, false);
+  });
+}
+int mfg_operator_vmult_add(mfg_operator op, void *dst, const void *src) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    op->apply(src, dst, true);
+  });
+}
+int mfg_operator_Tvmult(mfg_operator op, void *dst, const void *src) { return mfg_operator_vmult(op, dst, src); }
+int mfg_operator_Tvmult_add(mfg_operator op, void *dst, const void *src) {
+  return mfg_operator_vmult_add(op, dst, src);
+}
+
+int mfg_operator_compute_diagonal(mfg_operator op) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    op->compute_diagonal();
+  });
+}
+
+int mfg_operator_copy_diagonal(mfg_operator op, void *diag, void *inv_diag) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    MFG_REQUIRE(op->has_diagonal, MFG_ERR_STATE, "compute_diagonal() not called");
+    if (diag) MFG_HIP(hipMemcpyAsync(diag, op->diag.ptr, op->diag.bytes, hipMemcpyDeviceToDevice, op->stream));
+    if (inv_diag)
+      MFG_HIP(hipMemcpyAsync(inv_diag, op->inv_diag.ptr, op->inv_diag.bytes, hipMemcpyDeviceToDevice, op->stream));
+  });
+}
+
+int mfg_operator_el(mfg_operator op, int64_t row, int64_t col, double *value) {
+  return guarded([&] {
+    MFG_REQUIRE(op && value, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(row == col, MFG_ERR_ARGUMENT, "el() only provides diagonal entries");
+    MFG_REQUIRE(row >= 0 && row < op->n_dofs, MFG_ERR_ARGUMENT, "row out of range");
+    MFG_REQUIRE(op->has_diagonal, MFG_ERR_STATE, "compute_diagonal() not called");
+    if (op->precision == MFG_FP64) {
+      MFG_HIP(hipMemcpyAsync(value, op->diag.as<double>() + row, 8, hipMemcpyDeviceToHost, op->stream));
+      MFG_HIP(hipStreamSynchronize(op->stream));
+    } else {
+      float f = 0;
+      MFG_HIP(hipMemcpyAsync(&f, op->diag.as<float>() + row, 4, hipMemcpyDeviceToHost, op->stream));
+      MFG_HIP(hipStreamSynchronize(op->stream));
+      *value = f;
+    }
+  });
+}
+
+int mfg_operator_precondition_jacobi(mfg_operator op, void *dst, const void *src, double omega) {
+  return guarded([&] {
+    MFG_REQUIRE(op && dst && src, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(op->has_diagonal, MFG_ERR_STATE, "compute_diagonal() not called");
+    const int threads = 256, blocks = std::min(blocks_for(op->n_dofs, threads), 65536);
+    if (op->precision == MFG_FP64)
+      scaled_product_kernel<double><<<blocks, threads, 0, op->stream>>>((double *)dst, op->inv_diag.as<double>(),
+                                                                        (const double *)src, omega, op->n_dofs);
+    else
+      scaled_product_kernel<float><<<blocks, threads, 0, op->stream>>>((float *)dst, op->inv_diag.as<float>(),
+                                                                       (const float *)src, (float)omega, op->n_dofs);
+    MFG_HIP(hipGetLastError());
+  });
+}
+
+size_t mfg_operator_memory_consumption(mfg_operator op) { return op ? op->memory_consumption() : 0; }
+
+int mfg_operator_enable_timing(mfg_operator op, int32_t enable) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    op->timing = enable != 0;
+    op->events_used = 0;
+  });
+}
+
+int mfg_operator_timing(mfg_operator op, double *total_ms, int64_t *launches) {
+  return guarded([&] {
+    MFG_REQUIRE(op && total_ms && launches, MFG_ERR_ARGUMENT, "null argument");
+    op->timing_collect(*total_ms, *launches);
+  });
+}
+
+int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    if (n_fields) *n_fields = op->n_fields;
+    if (out) {
+      MFG_REQUIRE(op->ready, MFG_ERR_STATE, "no quadrature data yet");
+      MFG_HIP(hipMemcpyAsync(out, op->qd.ptr, op->qd.bytes, hipMemcpyDeviceToDevice, op->stream));
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,46 @@
+// Device operator object shared by the operator, multigrid and solver code.
+#pragma once
+#include "common.hpp"
+
+namespace mfg {
+
+struct OperatorData {
+  int kind = 0;  // 0 solid (NeoHookOperator), 1 heat (LaplaceOperator)
+  int precision = MFG_FP64;
+  int p = 1, nc = 1, n_cells = 0, n_fields = 0;
+  int64_t n_nodes = 0, n_dofs = 0, nqp = 0, n_constrained = 0;
+  hipStream_t stream = nullptr;
+  mfg_solid_params sp{};
+  double kappa = 0, c_1 = 0, lambda = 0, rho_alpha = 0, dt = 0;
+  bool ready = false, has_diagonal = false;
+
+  DeviceBuffer cell_nodes;   // uint32: node | constraint bits << 28
+  DeviceBuffer constrained;  // int32 sorted dof list
+  DeviceBuffer vertices;     // double n_cells*24
+  DeviceBuffer tab_shape, tab_colloc, tab_sq, tab_points, tab_weights;
+  DeviceBuffer qd;           // quadrature cache, field-major SoA
+  DeviceBuffer diag, inv_diag;
+  DeviceBuffer flag;         // int error flag
+  std::vector<int32_t> host_constrained;
+  // optional cell-kernel timing
+  bool timing = false;
+  mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  mutable size_t events_used = 0;
+  ~OperatorData();
+
+  void setup_layout(const mfg_level_desc &d);
+  void setup_solid();
+  void setup_heat(const mfg_level_desc &d, const double *coef);
+  void cache(const void *u, int u_precision);
+  void apply(const void *src, void *dst, bool add) const;
+  void compute_diagonal();
+  size_t memory_consumption() const;
+  size_t elem_size() const { return precision == MFG_FP64 ? 8 : 4; }
+  void timing_begin() const;
+  void timing_end() const;
+  void timing_collect(double &ms, int64_t &n);
+};
+
+}  // namespace mfg
+
+struct mfg_operator_t : mfg::OperatorData {};

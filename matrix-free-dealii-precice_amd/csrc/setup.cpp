@@ -1,0 +1,266 @@
+// Host-side set-up of libmfgpu: last-error storage, 1D finite-element tables
+// and the structured box mesh builder that produces the level layout
+// (cell->node maps, Dirichlet lists, cell vertices) the operators consume.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace mfg {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+// Legendre P_n, P_n' and P_n'' at t in (-1,1).
+static void legendre3(int n, double t, double &p, double &dp, double &ddp) {
+  double pm = 1.0, pc = t;
+  if (n == 0) { p = 1; dp = 0; ddp = 0; return; }
+  for (int k = 2; k <= n; ++k) {
+    double pn = ((2.0 * k - 1.0) * t * pc - (k - 1.0) * pm) / k;
+    pm = pc;
+    pc = pn;
+  }
+  p = pc;
+  const double s = 1.0 - t * t;
+  dp = n * (pm - t * pc) / s;
+  // Legendre equation: (1-t^2) P'' - 2 t P' + n(n+1) P = 0
+  ddp = (2.0 * t * dp - n * (n + 1.0) * p) / s;
+}
+
+std::vector<double> gauss_lobatto_points(int p) {
+  std::vector<double> x(p + 1);
+  x[0] = 0.0;
+  x[p] = 1.0;
+  // interior points: roots of P_p' by Newton, started from Chebyshev-Lobatto points
+  for (int i = 1; i < p; ++i) {
+    double t = -std::cos(M_PI * i / p);
+    for (int it = 0; it < 100; ++it) {
+      double P, dP, ddP;
+      legendre3(p, t, P, dP, ddP);
+      const double step = dP / ddP;
+      t -= step;
+      if (std::abs(step) < 1e-17) break;
+    }
+    x[i] = 0.5 * (1.0 + t);
+  }
+  for (int i = 0; i <= p / 2; ++i) {  // enforce the mirror symmetry exactly
+    const double a = 0.5 * (x[i] + (1.0 - x[p - i]));
+    x[i] = a;
+    x[p - i] = 1.0 - a;
+  }
+  if (p % 2 == 0) x[p / 2] = 0.5;
+  return x;
+}
+
+void gauss_points(int n, std::vector<double> &x, std::vector<double> &w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double t = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double P = 0, dP = 0, ddP = 0;
+    for (int it = 0; it < 100; ++it) {
+      legendre3(n, t, P, dP, ddP);
+      const double step = P / dP;
+      t -= step;
+      if (std::abs(step) < 1e-17) break;
+    }
+    legendre3(n, t, P, dP, ddP);
+    x[i] = 0.5 * (1.0 + t);
+    w[i] = 1.0 / ((1.0 - t * t) * dP * dP);
+  }
+}
+
+void lagrange_basis(const std::vector<double> &z, double x, double *val, double *der) {
+  const int m = (int)z.size();
+  // barycentric weights
+  std::vector<double> bw(m, 1.0);
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b)
+      if (a != b) bw[a] /= (z[a] - z[b]);
+  for (int a = 0; a < m; ++a) {
+    if (std::abs(x - z[a]) < 1e-15) {
+      // at a node: value is the Kronecker delta; derivative from the
+      // differentiation-matrix formula D_ab = (w_b/w_a)/(z_a - z_b)
+      for (int b = 0; b < m; ++b) val[b] = (a == b) ? 1.0 : 0.0;
+      double diag = 0;
+      for (int b = 0; b < m; ++b)
+        if (b != a) {
+          der[b] = (bw[b] / bw[a]) / (z[a] - z[b]);
+          diag -= der[b];
+        }
+      der[a] = diag;
+      return;
+    }
+  }
+  // l(x) = prod (x - z_b); l_a(x) = l(x) w_a / (x - z_a)
+  double l = 1.0;
+  for (int b = 0; b < m; ++b) l *= (x - z[b]);
+  double s = 0.0;
+  for (int a = 0; a < m; ++a) s += 1.0 / (x - z[a]);
+  for (int a = 0; a < m; ++a) {
+    val[a] = l * bw[a] / (x - z[a]);
+    // l_a'(x) = l_a(x) * (sum_{b != a} 1/(x - z_b))
+    der[a] = val[a] * (s - 1.0 / (x - z[a]));
+  }
+}
+
+double uniform_pm1_host(uint64_t seed, uint64_t index);
+
+Tables1D make_tables(int p) {
+  Tables1D t;
+  t.N = p + 1;
+  const int N = t.N;
+  std::vector<double> z = gauss_lobatto_points(p);
+  std::vector<double> q, w;
+  gauss_points(N, q, w);
+  t.points = q;
+  t.weight = w;
+  t.shape.assign(N * N, 0.0);
+  t.colloc.assign(N * N, 0.0);
+  std::vector<double> val(N), der(N);
+  for (int i = 0; i < N; ++i) {
+    lagrange_basis(z, q[i], val.data(), der.data());
+    for (int a = 0; a < N; ++a) t.shape[i * N + a] = val[a];
+    lagrange_basis(q, q[i], val.data(), der.data());
+    for (int r = 0; r < N; ++r) t.colloc[i * N + r] = der[r];
+  }
+  return t;
+}
+
+}  // namespace mfg
+
+// ------------------------------------------------------------ box meshes ---
+struct mfg_mesh_t {
+  int p = 1, nc = 1;
+  int n[3] = {1, 1, 1}, NN[3] = {2, 2, 2};
+  double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  int64_t n_nodes = 0;
+  std::vector<int32_t> cell_nodes;
+  std::vector<double> vertices;
+  std::vector<int32_t> constrained;
+  std::vector<double> node_x;
+};
+
+using namespace mfg;
+
+extern "C" {
+
+const char *mfg_last_error(void) { return g_last_error.c_str(); }
+const char *mfg_version(void) { return "mfgpu 0.1 (gfx950)"; }
+
+int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n[3], int32_t degree,
+                        int32_t n_components, int32_t face_mask, int32_t comp_mask, mfg_mesh *out) {
+  return guarded([&] {
+    MFG_REQUIRE(out && lo && hi && n, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(degree >= 1 && degree <= kMaxDegree, MFG_ERR_ARGUMENT, "degree must be in 1..6");
+    MFG_REQUIRE(n_components == 1 || n_components == 3, MFG_ERR_ARGUMENT, "n_components must be 1 or 3");
+    auto m = new mfg_mesh_t();
+    m->p = degree;
+    m->nc = n_components;
+    const int p = degree, N = p + 1, N3 = N * N * N;
+    for (int d = 0; d < 3; ++d) {
+      if (n[d] < 1 || !(hi[d] > lo[d])) { delete m; throw Error(MFG_ERR_ARGUMENT, "bad box extents"); }
+      m->n[d] = n[d];
+      m->lo[d] = lo[d];
+      m->hi[d] = hi[d];
+      m->NN[d] = n[d] * p + 1;
+    }
+    m->n_nodes = (int64_t)m->NN[0] * m->NN[1] * m->NN[2];
+    if (m->n_nodes >= ((int64_t)1 << kNodeBits)) { delete m; throw Error(MFG_ERR_ARGUMENT, "mesh too large"); }
+    const int64_t n_cells = (int64_t)n[0] * n[1] * n[2];
+    m->cell_nodes.resize(n_cells * N3);
+    m->vertices.resize(n_cells * 24);
+    const std::vector<double> z = gauss_lobatto_points(p);
+    double h[3];
+    for (int d = 0; d < 3; ++d) h[d] = (hi[d] - lo[d]) / n[d];
+    for (int64_t c = 0; c < n_cells; ++c) {
+      const int cx = (int)(c % n[0]), cy = (int)((c / n[0]) % n[1]), cz = (int)(c / ((int64_t)n[0] * n[1]));
+      int32_t *cn = &m->cell_nodes[c * N3];
+      for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j) {
+          const int64_t row = (int64_t)(cx * p) + (int64_t)m->NN[0] * ((cy * p + j) + (int64_t)m->NN[1] * (cz * p + k));
+          for (int i = 0; i < N; ++i) cn[i + N * (j + N * k)] = (int32_t)(row + i);
+        }
+      for (int v = 0; v < 8; ++v) {
+        const int o[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+        const int cc[3] = {cx, cy, cz};
+        for (int d = 0; d < 3; ++d) m->vertices[c * 24 + v * 3 + d] = lo[d] + (cc[d] + o[d]) * h[d];
+      }
+    }
+    // node coordinates from the cell-local GLL support points
+    m->node_x.resize(m->n_nodes * 3);
+    std::vector<double> line[3];
+    for (int d = 0; d < 3; ++d) {
+      line[d].resize(m->NN[d]);
+      for (int I = 0; I < m->NN[d]; ++I) {
+        const int c = std::min(I / p, n[d] - 1);
+        line[d][I] = lo[d] + (c + z[I - c * p]) * h[d];
+      }
+    }
+    for (int64_t id = 0; id < m->n_nodes; ++id) {
+      const int I = (int)(id % m->NN[0]), J = (int)((id / m->NN[0]) % m->NN[1]),
+                K = (int)(id / ((int64_t)m->NN[0] * m->NN[1]));
+      m->node_x[id * 3 + 0] = line[0][I];
+      m->node_x[id * 3 + 1] = line[1][J];
+      m->node_x[id * 3 + 2] = line[2][K];
+    }
+    // homogeneous Dirichlet dofs on the selected faces; sorted by construction
+    for (int64_t id = 0; id < m->n_nodes; ++id) {
+      const int I = (int)(id % m->NN[0]), J = (int)((id / m->NN[0]) % m->NN[1]),
+                K = (int)(id / ((int64_t)m->NN[0] * m->NN[1]));
+      const bool on = ((face_mask & 1) && I == 0) || ((face_mask & 2) && I == m->NN[0] - 1) ||
+                      ((face_mask & 4) && J == 0) || ((face_mask & 8) && J == m->NN[1] - 1) ||
+                      ((face_mask & 16) && K == 0) || ((face_mask & 32) && K == m->NN[2] - 1);
+      if (!on) continue;
+      for (int c = 0; c < n_components; ++c)
+        if (comp_mask & (1 << c)) m->constrained.push_back((int32_t)(id * n_components + c));
+    }
+    *out = m;
+  });
+}
+
+int mfg_box_mesh_destroy(mfg_mesh mesh) {
+  delete mesh;
+  return MFG_OK;
+}
+
+int mfg_box_mesh_describe(mfg_mesh m, mfg_level_desc *d) {
+  return guarded([&] {
+    MFG_REQUIRE(m && d, MFG_ERR_ARGUMENT, "null argument");
+    d->degree = m->p;
+    d->n_q_1d = m->p + 1;
+    d->n_components = m->nc;
+    d->n_cells = (int32_t)(m->vertices.size() / 24);
+    d->n_nodes = m->n_nodes;
+    d->cell_nodes = m->cell_nodes.data();
+    d->cell_vertices = m->vertices.data();
+    d->constrained_dofs = m->constrained.data();
+    d->n_constrained = (int64_t)m->constrained.size();
+  });
+}
+
+int mfg_box_mesh_fill_u0(mfg_mesh m, double amp, double noise, uint64_t seed, double *out) {
+  return guarded([&] {
+    MFG_REQUIRE(m && out, MFG_ERR_ARGUMENT, "null argument");
+    std::vector<char> con((size_t)m->n_nodes * m->nc, 0);
+    for (int32_t c : m->constrained) con[c] = 1;
+    for (int64_t node = 0; node < m->n_nodes; ++node) {
+      const double *x = &m->node_x[node * 3];
+      const double base = amp * std::sin(M_PI * x[0]) * std::sin(M_PI * x[1]) * std::sin(M_PI * x[2]);
+      for (int c = 0; c < m->nc; ++c) {
+        const int64_t dof = node * m->nc + c;
+        out[dof] = con[dof] ? 0.0 : base + noise * uniform_pm1_host(seed, (uint64_t)dof);
+      }
+    }
+  });
+}
+
+int mfg_box_mesh_node_coords(mfg_mesh m, double *out) {
+  return guarded([&] {
+    MFG_REQUIRE(m && out, MFG_ERR_ARGUMENT, "null argument");
+    std::memcpy(out, m->node_x.data(), m->node_x.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,287 @@
+"""Python binding of libmfgpu's C ABI (include/mfgpu.h).
+
+The library is the product; this module only loads it with ctypes, checks
+status codes (raising MfgError, the counterpart of deal.II's AssertThrow) and
+offers thin classes whose method names follow the reference operators
+(NeoHookOperator / LaplaceOperator: vmult, vmult_add, Tvmult, cache,
+compute_diagonal, el, precondition_Jacobi, memory_consumption, m, n).
+Device vectors are torch tensors on the GPU; torch provides memory and
+streams, never compute on this path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfgpu.so")
+
+FP64, FP32 = 0, 1
+CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
+
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+class MfgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mfgpu error {code}: {msg}")
+        self.code = code
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [("degree", ctypes.c_int32), ("n_q_1d", ctypes.c_int32), ("n_components", ctypes.c_int32),
+                ("n_cells", ctypes.c_int32), ("n_nodes", ctypes.c_int64), ("cell_nodes", _c_i32p),
+                ("cell_vertices", _c_dp), ("constrained_dofs", _c_i32p), ("n_constrained", ctypes.c_int64)]
+
+
+class SolidParams(ctypes.Structure):
+    _fields_ = [("mu", ctypes.c_double), ("nu", ctypes.c_double), ("rho", ctypes.c_double),
+                ("alpha_1", ctypes.c_double), ("formulation", ctypes.c_int32), ("caching", ctypes.c_int32)]
+
+
+_LIB = None
+
+# name -> (restype, argtypes); the exported surface of include/mfgpu.h
+SIGNATURES = {
+    "mfg_last_error": (ctypes.c_char_p, []),
+    "mfg_version": (ctypes.c_char_p, []),
+    "mfg_box_mesh_create": (ctypes.c_int, [_c_dp, _c_dp, _c_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "mfg_box_mesh_destroy": (ctypes.c_int, [_vp]),
+    "mfg_box_mesh_describe": (ctypes.c_int, [_vp, ctypes.POINTER(LevelDesc)]),
+    "mfg_box_mesh_node_coords": (ctypes.c_int, [_vp, _c_dp]),
+    "mfg_solid_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), ctypes.POINTER(SolidParams), ctypes.c_int32,
+                                         ctypes.POINTER(_vp)]),
+    "mfg_heat_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), ctypes.c_double, _c_dp, ctypes.c_int32,
+                                        ctypes.POINTER(_vp)]),
+    "mfg_operator_destroy": (ctypes.c_int, [_vp]),
+    "mfg_operator_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "mfg_operator_m": (ctypes.c_int64, [_vp]),
+    "mfg_operator_n": (ctypes.c_int64, [_vp]),
+    "mfg_operator_precision": (ctypes.c_int32, [_vp]),
+    "mfg_solid_cache": (ctypes.c_int, [_vp, _vp, ctypes.c_int32]),
+    "mfg_operator_vmult": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_operator_vmult_add": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_operator_Tvmult": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_operator_Tvmult_add": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_operator_compute_diagonal": (ctypes.c_int, [_vp]),
+    "mfg_operator_copy_diagonal": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_operator_el": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _c_dp]),
+    "mfg_operator_precondition_jacobi": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double]),
+    "mfg_operator_memory_consumption": (ctypes.c_size_t, [_vp]),
+    "mfg_operator_qp_data": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32)]),
+    "mfg_operator_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_operator_timing": (ctypes.c_int, [_vp, _c_dp, ctypes.POINTER(ctypes.c_int64)]),
+    "mfg_fill_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
+    "mfg_box_mesh_fill_u0": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _c_dp]),
+}
+
+
+def fill_uniform(seed, t, stream=None):
+    """t <- seeded U[-1,1] (splitmix64), generated on the device."""
+    import torch
+    prec = FP64 if t.dtype == torch.float64 else FP32
+    s = None if stream is None else ctypes.c_void_p(int(stream.cuda_stream))
+    check(lib().mfg_fill_uniform(seed, t.numel(), prec, _ptr(t), s))
+    return t
+
+
+def lib():
+    """Load libmfgpu.so; raises if the HIP library has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status):
+    if status != 0:
+        raise MfgError(status, lib().mfg_last_error().decode())
+
+
+def _ptr(t):
+    """Device (or host numpy) pointer of a contiguous tensor/array."""
+    if hasattr(t, "data_ptr"):
+        assert t.is_contiguous()
+        return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.ctypes.data)
+
+
+def _dtype_of(precision):
+    import torch
+    return torch.float64 if precision == FP64 else torch.float32
+
+
+class BoxMesh:
+    """Structured box mesh level (cells, nodes, Dirichlet dofs) built on the host."""
+
+    def __init__(self, lo, hi, n_cells, degree, n_components, face_mask=1, comp_mask=7):
+        import numpy as np
+        self._lo = np.ascontiguousarray(lo, dtype=np.float64)
+        self._hi = np.ascontiguousarray(hi, dtype=np.float64)
+        self._n = np.ascontiguousarray(n_cells, dtype=np.int32)
+        h = _vp()
+        check(lib().mfg_box_mesh_create(self._lo.ctypes.data_as(_c_dp), self._hi.ctypes.data_as(_c_dp),
+                                         self._n.ctypes.data_as(_c_i32p), degree, n_components, face_mask,
+                                         comp_mask, ctypes.byref(h)))
+        self.h = h
+        self.desc = LevelDesc()
+        check(lib().mfg_box_mesh_describe(self.h, ctypes.byref(self.desc)))
+        self.degree, self.n_components = degree, n_components
+        self.n_cells_dir = list(n_cells)
+        self.n_cells = self.desc.n_cells
+        self.n_nodes = self.desc.n_nodes
+        self.n_dofs = self.n_nodes * n_components
+        self.n_constrained = self.desc.n_constrained
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mfg_box_mesh_destroy(self.h)
+            self.h = None
+
+    def cell_nodes(self):
+        import numpy as np
+        n = self.n_cells * (self.degree + 1) ** 3
+        return np.ctypeslib.as_array(self.desc.cell_nodes, shape=(n,)).copy()
+
+    def constrained(self):
+        import numpy as np
+        if self.n_constrained == 0:
+            return np.zeros(0, dtype=np.int32)
+        return np.ctypeslib.as_array(self.desc.constrained_dofs, shape=(self.n_constrained,)).copy()
+
+    def cell_vertices(self):
+        import numpy as np
+        return np.ctypeslib.as_array(self.desc.cell_vertices, shape=(self.n_cells * 24,)).copy()
+
+    def node_coords(self):
+        import numpy as np
+        out = np.zeros(self.n_nodes * 3)
+        check(lib().mfg_box_mesh_node_coords(self.h, out.ctypes.data_as(_c_dp)))
+        return out.reshape(-1, 3)
+
+    def u0(self, amp=1e-3, noise=1e-5, seed=0):
+        """Linearisation point: amp sin(pi x) sin(pi y) sin(pi z) + noise U[-1,1], host array."""
+        import numpy as np
+        out = np.zeros(self.n_dofs)
+        check(lib().mfg_box_mesh_fill_u0(self.h, amp, noise, seed, out.ctypes.data_as(_c_dp)))
+        return out
+
+
+class _Operator:
+    def __init__(self, handle, precision, keep):
+        self.h = handle
+        self.precision = precision
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mfg_operator_destroy(self.h)
+            self.h = None
+
+    @property
+    def dtype(self):
+        return _dtype_of(self.precision)
+
+    def m(self):
+        return lib().mfg_operator_m(self.h)
+
+    def n(self):
+        return lib().mfg_operator_n(self.h)
+
+    def set_stream(self, stream):
+        check(lib().mfg_operator_set_stream(self.h, ctypes.c_void_p(int(stream.cuda_stream))))
+
+    def vmult(self, dst, src):
+        check(lib().mfg_operator_vmult(self.h, _ptr(dst), _ptr(src)))
+
+    def vmult_add(self, dst, src):
+        check(lib().mfg_operator_vmult_add(self.h, _ptr(dst), _ptr(src)))
+
+    def Tvmult(self, dst, src):
+        check(lib().mfg_operator_Tvmult(self.h, _ptr(dst), _ptr(src)))
+
+    def Tvmult_add(self, dst, src):
+        check(lib().mfg_operator_Tvmult_add(self.h, _ptr(dst), _ptr(src)))
+
+    def compute_diagonal(self):
+        check(lib().mfg_operator_compute_diagonal(self.h))
+
+    def diagonal(self):
+        import torch
+        d = torch.empty(self.m(), dtype=self.dtype, device="cuda")
+        check(lib().mfg_operator_copy_diagonal(self.h, _ptr(d), None))
+        return d
+
+    def get_matrix_diagonal_inverse(self):
+        import torch
+        d = torch.empty(self.m(), dtype=self.dtype, device="cuda")
+        check(lib().mfg_operator_copy_diagonal(self.h, None, _ptr(d)))
+        return d
+
+    def el(self, row, col):
+        v = ctypes.c_double(0)
+        check(lib().mfg_operator_el(self.h, row, col, ctypes.byref(v)))
+        return v.value
+
+    def precondition_Jacobi(self, dst, src, omega):
+        check(lib().mfg_operator_precondition_jacobi(self.h, _ptr(dst), _ptr(src), omega))
+
+    def memory_consumption(self):
+        return lib().mfg_operator_memory_consumption(self.h)
+
+    def enable_timing(self, enable=True):
+        check(lib().mfg_operator_enable_timing(self.h, 1 if enable else 0))
+
+    def timing(self):
+        """(summed cell-kernel ms, launches) since the last call."""
+        ms, n = ctypes.c_double(0), ctypes.c_int64(0)
+        check(lib().mfg_operator_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def qp_data(self):
+        import torch
+        nf = ctypes.c_int32(0)
+        check(lib().mfg_operator_qp_data(self.h, None, ctypes.byref(nf)))
+        out = torch.empty(nf.value * self.n_qp, dtype=self.dtype, device="cuda")
+        check(lib().mfg_operator_qp_data(self.h, _ptr(out), ctypes.byref(nf)))
+        return out.view(nf.value, self.n_qp)
+
+
+class NeoHookOperator(_Operator):
+    """Device counterpart of NeoHookOperator<3, Number> (mf_nh_operator.h:99-237)."""
+
+    def __init__(self, mesh, mu, nu, rho, alpha_1, formulation=1, caching="tensor2", precision=FP64):
+        prm = SolidParams(mu, nu, rho, alpha_1, formulation, CACHING[caching])
+        h = _vp()
+        check(lib().mfg_solid_create(ctypes.byref(mesh.desc), ctypes.byref(prm), precision, ctypes.byref(h)))
+        super().__init__(h, precision, mesh)
+        self.n_qp = mesh.n_cells * (mesh.degree + 1) ** 3
+
+    def cache(self, u):
+        import torch
+        prec = FP64 if u.dtype == torch.float64 else FP32
+        check(lib().mfg_solid_cache(self.h, _ptr(u), prec))
+
+
+class LaplaceOperator(_Operator):
+    """Device counterpart of LaplaceOperator<3, number> (heat_transfer.h:81-250): M + dt*c*K."""
+
+    def __init__(self, mesh, dt, coefficient=None, precision=FP64):
+        import numpy as np
+        coef = None if coefficient is None else np.ascontiguousarray(coefficient, dtype=np.float64)
+        h = _vp()
+        check(lib().mfg_heat_create(ctypes.byref(mesh.desc), dt,
+                                    None if coef is None else coef.ctypes.data_as(_c_dp), precision,
+                                    ctypes.byref(h)))
+        super().__init__(h, precision, mesh)
+        self.n_qp = mesh.n_cells * (mesh.degree + 1) ** 3

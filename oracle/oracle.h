@@ -1,0 +1,160 @@
+/*
+ * CPU oracle: a host restatement of the reference's matrix-free operator and
+ * MG-preconditioned CG path, written from the mathematics the reference code
+ * states (mf_nh_operator.h, material.h, heat_transfer.h, mf_elasticity.h).
+ *
+ * This library is TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it, and only as the checker (or the
+ * timed CPU baseline); the product path never calls into it.
+ *
+ * Pinning status of every function is stated next to its declaration:
+ *   [pinned: ...] names the independent check(s) in tests/ that would fail on
+ *   a dropped term, a wrong sign/index or a transposed operand.
+ *
+ * All arrays are host arrays.  DoF numbering: dof = n_components*node + comp,
+ * nodes numbered lexicographically over the structured box
+ * (I + NX*(J + NY*K)), cells lexicographic (cx + nx*(cy + ny*cz)), cell-local
+ * node order lexicographic (a + N*(b + N*c)), N = degree+1.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- 1D FE --- */
+/* Gauss-Lobatto support points of FE_Q(p) on [0,1] (p+1 points).
+ * [pinned: closed forms for p<=4, endpoint/symmetry, Lagrange partition of
+ *  unity; tests/test_oracle_fe.py] */
+void orc_gll_points(int p, double *x);
+/* QGauss(n) points/weights on [0,1].
+ * [pinned: closed forms n<=3, exact integration of x^k, k<2n] */
+void orc_gauss_rule(int n, double *x, double *w);
+/* Lagrange basis on nodes z[0..m-1] evaluated at x[0..nx-1]:
+ * val[i*m+a] = l_a(x_i), der[i*m+a] = l_a'(x_i).
+ * [pinned: derivative vs. finite differences, reproduction of monomials] */
+void orc_lagrange(int m, const double *z, int nx, const double *x, double *val,
+                  double *der);
+
+/* --------------------------------------------------------------- meshes --- */
+/* Structured box [lo,hi] with n[0]xn[1]xn[2] hex cells, FE_Q(p)^nc on GLL
+ * nodes.  Dirichlet faces: bit f of face_mask selects face f in the
+ * deal.II colorize numbering (0:x=lo,1:x=hi,2:y=lo,3:y=hi,4:z=lo,5:z=hi);
+ * comp_mask selects constrained components.
+ * [pinned: closed-form node coordinates and constrained counts
+ *  3*(ny*p+1)*(nz*p+1); tests/test_oracle_mesh.py] */
+void *orc_mesh_create(const double *lo, const double *hi, const int *n, int p,
+                      int nc, int face_mask, int comp_mask);
+void orc_mesh_destroy(void *m);
+/* out[0..5] = n_cells, n_nodes, n_dofs, n_constrained, nodes_per_cell, nc */
+void orc_mesh_sizes(void *m, int64_t *out);
+void orc_mesh_cell_nodes(void *m, int32_t *out);    /* n_cells*(p+1)^3 */
+void orc_mesh_constrained(void *m, int32_t *out);   /* sorted dof list */
+void orc_mesh_node_coords(void *m, double *out);    /* n_nodes*3 */
+void orc_mesh_cell_vertices(void *m, double *out);  /* n_cells*8*3 */
+
+/* Nodal interpolation of the function u0(x)=amp*sin(pi x)sin(pi y)sin(pi z)
+ * per component plus noise*U[-1,1] from splitmix64(seed, dof); constrained
+ * entries zeroed.  Used to generate the linearisation point. */
+void orc_fill_u0(void *m, double amp, double noise, uint64_t seed, double *u);
+/* splitmix64-based U[-1,1] vector of length n (constrained-agnostic). */
+void orc_fill_uniform(uint64_t seed, int64_t n, double *v);
+
+/* ------------------------------------------------------ solid operator --- */
+/* caching: 0 scalar_referential, 1 tensor2, 2 tensor4, 3 tensor4_ns
+ * formulation: 0 (volumetric/isochoric split) or 1 (one-field, default).
+ * Material: mu, nu, rho (material.h:78-94); alpha_1 = 1/(beta dt^2).
+ * [pinned: brute-force element assembly (orc_solid_vmult_bruteforce),
+ *  agreement of the four cachings, u=0 reduction to linear elasticity with
+ *  Lame pair (mu, 2 lambda) + rho*alpha_1*M (closed-form mass 1^T M 1),
+ *  symmetry, finite differences of orc_solid_residual] */
+void *orc_solid_create(void *mesh, int n_q, double mu, double nu, double rho,
+                       double alpha_1, int formulation, int caching);
+void orc_solid_destroy(void *op);
+/* cache(): linearise at displacement u (mf_nh_operator.h:392-523). Returns 0,
+ * or -1 if det F <= 0 at some quadrature point. */
+int orc_solid_cache(void *op, const double *u);
+/* dst = A src with identity rows on constrained dofs (mf_nh_operator.h:575-625) */
+void orc_solid_vmult(void *op, double *dst, const double *src);
+/* Independent dense element-matrix path (no sum factorisation, no cache):
+ * F, tau and the tangent are recomputed from u with plain quadrature loops. */
+void orc_solid_vmult_bruteforce(void *op, double *dst, const double *src);
+/* diagonal with constrained entries = 1 (mf_nh_operator.h:1023-1060) */
+void orc_solid_diagonal(void *op, double *diag);
+/* tensor2 formulation-1 quadrature data in the device layout:
+ * out[f*n_qp + cell*nq3 + q], f = 0..17: Jinv_cur[f0][e] row-major (9),
+ * JxW_cur, c2 = 2(mu-2 lambda lnJ)/J, c1 = 1/J, T=tau/J (xx,yy,zz,xy,xz,yz) */
+void orc_solid_tensor2_data(void *op, double *out);
+/* Residual R(u, a) = -int tau F^{-T}:Grad w + int rho (b - a).w, constrained
+ * entries zeroed (mf_elasticity.h:1764-1846, 1905-1907).  body = constant
+ * body force (3). */
+void orc_solid_residual(void *op, const double *u, const double *acc,
+                        const double *body, double *R);
+
+/* ------------------------------------------------------- heat operator --- */
+/* A = M + dt * c(x) K on scalar FE_Q(p) (heat_transfer.h:234-250).
+ * coef: per (cell, q) conductivity table, or NULL for c == 1.
+ * [pinned: brute force element assembly; closed form 1^T M 1 = |Omega|;
+ *  exact reproduction of a quadratic manufactured solution] */
+void *orc_heat_create(void *mesh, int n_q, double dt, const double *coef);
+void orc_heat_destroy(void *op);
+void orc_heat_vmult(void *op, double *dst, const double *src);
+void orc_heat_vmult_bruteforce(void *op, double *dst, const double *src);
+void orc_heat_diagonal(void *op, double *diag);
+/* Unconstrained apply (no constraint handling at all): used for the
+ * inhomogeneous-Dirichlet lifting of the right-hand side. */
+void orc_heat_vmult_noconstraints(void *op, double *dst, const double *src);
+/* mesh an operator was created on (kind 0 solid, 1 heat) */
+void *orc_operator_mesh(int kind, void *op);
+/* load vector: int f phi_i dx for constant f */
+void orc_heat_load(void *op, double f, double *b);
+
+/* ------------------------------------------------------- MG hierarchy --- */
+/* Transfer between nested boxes (fine = coarse refined once).
+ * prolongate_add: fine += P coarse (coarse constrained read as 0, fine
+ * constrained untouched); restrict_add: coarse += P^T fine (fine constrained
+ * read as 0, coarse constrained untouched); interpolate: coarse = nodal
+ * interpolant of the fine FE function.
+ * [pinned: polynomial reproduction, <P c, f> = <c, P^T f>, interpolate o
+ *  prolongate = identity on coarse space] */
+void orc_prolongate_add(void *coarse_mesh, void *fine_mesh, const double *c,
+                        double *f);
+void orc_restrict_add(void *coarse_mesh, void *fine_mesh, const double *f,
+                      double *c);
+void orc_interpolate(void *coarse_mesh, void *fine_mesh, const double *f,
+                     double *c);
+
+/* MG-preconditioned CG on a level hierarchy of solid or heat operators.
+ * ops[l], meshes[l] for l = 0..L (coarse..fine).  kind: 0 solid, 1 heat.
+ * Chebyshev: degree, smoothing_range, eig_cg iterations on levels >0;
+ * coarse: 0 -> Chebyshev "as solver" (range 1e-3, degree from the estimate,
+ * eig iterations = m()), 1 -> CG to relative 1e-3 preconditioned by the
+ * level-0 Chebyshev.  Returns iterations (or -1 on no convergence).
+ * x is the initial guess on entry (zeroed by the caller in practice).
+ * [pinned: convergence to the requested relative residual, rhs = A x_true
+ *  recovers x_true; Chebyshev/eigenvalue bounds vs. dense eigen-solve on
+ *  tiny levels] */
+typedef struct {
+  int cheb_degree;
+  double smoothing_range;
+  int eig_cg_iterations;
+  int coarse_kind;
+  double tol_rel;
+  int max_iter;
+} orc_mg_params;
+int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops,
+                    const orc_mg_params *prm, const double *b, double *x,
+                    double *final_rel_residual);
+/* plain / Jacobi-preconditioned CG (omega scales the inverse diagonal) */
+int orc_cg_solve(int kind, void *op, int precond, double omega, const double *b,
+                 double *x, double tol_rel, int max_iter, double *final_rel);
+/* Chebyshev smoother diagnostics on one level: returns the Lanczos max/min
+ * eigenvalue estimates (of D^-1 A) from n_it CG iterations started on the
+ * documented initial vector. */
+void orc_cheb_eigenvalues(int kind, void *op, void *mesh, int n_it,
+                          double *lmax, double *lmin);
+
+#ifdef __cplusplus
+}
+#endif

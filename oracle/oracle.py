@@ -1,0 +1,306 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker / CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class MGParams(ctypes.Structure):
+    _fields_ = [("cheb_degree", ctypes.c_int), ("smoothing_range", ctypes.c_double),
+                ("eig_cg_iterations", ctypes.c_int), ("coarse_kind", ctypes.c_int),
+                ("tol_rel", ctypes.c_double), ("max_iter", ctypes.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        sig = {
+            "orc_gll_points": (None, [ctypes.c_int, _dp]),
+            "orc_gauss_rule": (None, [ctypes.c_int, _dp, _dp]),
+            "orc_lagrange": (None, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]),
+            "orc_mesh_create": (_vp, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "orc_mesh_destroy": (None, [_vp]),
+            "orc_mesh_sizes": (None, [_vp, _lp]),
+            "orc_mesh_cell_nodes": (None, [_vp, _ip]),
+            "orc_mesh_constrained": (None, [_vp, _ip]),
+            "orc_mesh_node_coords": (None, [_vp, _dp]),
+            "orc_mesh_cell_vertices": (None, [_vp, _dp]),
+            "orc_fill_u0": (None, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp]),
+            "orc_fill_uniform": (None, [ctypes.c_uint64, ctypes.c_int64, _dp]),
+            "orc_solid_create": (_vp, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+            "orc_solid_destroy": (None, [_vp]),
+            "orc_solid_cache": (ctypes.c_int, [_vp, _dp]),
+            "orc_solid_vmult": (None, [_vp, _dp, _dp]),
+            "orc_solid_vmult_bruteforce": (None, [_vp, _dp, _dp]),
+            "orc_solid_diagonal": (None, [_vp, _dp]),
+            "orc_solid_tensor2_data": (None, [_vp, _dp]),
+            "orc_solid_residual": (None, [_vp, _dp, _dp, _dp, _dp]),
+            "orc_heat_create": (_vp, [_vp, ctypes.c_int, ctypes.c_double, _dp]),
+            "orc_heat_destroy": (None, [_vp]),
+            "orc_heat_vmult": (None, [_vp, _dp, _dp]),
+            "orc_heat_vmult_bruteforce": (None, [_vp, _dp, _dp]),
+            "orc_heat_vmult_noconstraints": (None, [_vp, _dp, _dp]),
+            "orc_heat_diagonal": (None, [_vp, _dp]),
+            "orc_heat_load": (None, [_vp, ctypes.c_double, _dp]),
+            "orc_operator_mesh": (_vp, [ctypes.c_int, _vp]),
+            "orc_prolongate_add": (None, [_vp, _vp, _dp, _dp]),
+            "orc_restrict_add": (None, [_vp, _vp, _dp, _dp]),
+            "orc_interpolate": (None, [_vp, _vp, _dp, _dp]),
+            "orc_mg_cg_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
+                                               ctypes.POINTER(_vp), ctypes.POINTER(MGParams), _dp, _dp, _dp]),
+            "orc_cg_solve": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                            ctypes.c_double, ctypes.c_int, _dp]),
+            "orc_cheb_eigenvalues": (None, [ctypes.c_int, _vp, _vp, ctypes.c_int, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+# material and Newmark constants of the benchmark configurations
+E_MOD, NU, RHO, BETA, DT = 1.0e6, 0.3, 1000.0, 0.25, 1.0e-3
+MU = E_MOD / (2.0 * (1.0 + NU))
+ALPHA1 = 1.0 / (BETA * DT * DT)
+
+
+def gll_points(p):
+    x = np.zeros(p + 1)
+    lib().orc_gll_points(p, dptr(x))
+    return x
+
+
+def gauss_rule(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_gauss_rule(n, dptr(x), dptr(w))
+    return x, w
+
+
+def lagrange(z, x):
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    val = np.zeros((len(x), len(z)))
+    der = np.zeros((len(x), len(z)))
+    lib().orc_lagrange(len(z), dptr(z), len(x), dptr(x), dptr(val), dptr(der))
+    return val, der
+
+
+def fill_uniform(seed, n):
+    v = np.zeros(n)
+    lib().orc_fill_uniform(seed, n, dptr(v))
+    return v
+
+
+class Mesh:
+    def __init__(self, lo, hi, n, p, nc, face_mask=1, comp_mask=7):
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        self.lo, self.hi, self.n, self.p, self.nc = lo, hi, list(n), p, nc
+        self.face_mask, self.comp_mask = face_mask, comp_mask
+        self.h = lib().orc_mesh_create(dptr(lo), dptr(hi), iptr(nn), p, nc, face_mask, comp_mask)
+        s = np.zeros(6, dtype=np.int64)
+        lib().orc_mesh_sizes(self.h, s.ctypes.data_as(_lp))
+        self.n_cells, self.n_nodes, self.n_dofs, self.n_constrained, self.nodes_per_cell = (int(v) for v in s[:5])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_destroy(self.h)
+            self.h = None
+
+    def cell_nodes(self):
+        out = np.zeros(self.n_cells * self.nodes_per_cell, dtype=np.int32)
+        lib().orc_mesh_cell_nodes(self.h, iptr(out))
+        return out
+
+    def constrained(self):
+        out = np.zeros(self.n_constrained, dtype=np.int32)
+        lib().orc_mesh_constrained(self.h, iptr(out))
+        return out
+
+    def node_coords(self):
+        out = np.zeros(self.n_nodes * 3)
+        lib().orc_mesh_node_coords(self.h, dptr(out))
+        return out.reshape(-1, 3)
+
+    def cell_vertices(self):
+        out = np.zeros(self.n_cells * 24)
+        lib().orc_mesh_cell_vertices(self.h, dptr(out))
+        return out
+
+    def u0(self, amp=1e-3, noise=1e-5, seed=0):
+        u = np.zeros(self.n_dofs)
+        lib().orc_fill_u0(self.h, amp, noise, seed, dptr(u))
+        return u
+
+
+class Solid:
+    KIND = 0
+
+    def __init__(self, mesh, n_q=None, mu=MU, nu=NU, rho=RHO, alpha1=ALPHA1, formulation=1, caching=1):
+        self.mesh = mesh
+        n_q = mesh.p + 1 if n_q is None else n_q
+        self.h = lib().orc_solid_create(mesh.h, n_q, mu, nu, rho, alpha1, formulation, caching)
+        if not self.h:
+            raise ValueError("unsupported solid operator configuration")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_solid_destroy(self.h)
+            self.h = None
+
+    def cache(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if lib().orc_solid_cache(self.h, dptr(u)) != 0:
+            raise ValueError("det_F is not positive")
+
+    def vmult(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros_like(src)
+        lib().orc_solid_vmult(self.h, dptr(dst), dptr(src))
+        return dst
+
+    def vmult_bruteforce(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros_like(src)
+        lib().orc_solid_vmult_bruteforce(self.h, dptr(dst), dptr(src))
+        return dst
+
+    def diagonal(self):
+        d = np.zeros(self.mesh.n_dofs)
+        lib().orc_solid_diagonal(self.h, dptr(d))
+        return d
+
+    def tensor2_data(self):
+        nq = self.mesh.n_cells * (self.mesh.p + 1) ** 3
+        out = np.zeros(18 * nq)
+        lib().orc_solid_tensor2_data(self.h, dptr(out))
+        return out.reshape(18, nq)
+
+    def residual(self, u, acc, body=(0.0, 0.0, 0.0)):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        acc = np.ascontiguousarray(acc, dtype=np.float64)
+        b = np.ascontiguousarray(body, dtype=np.float64)
+        R = np.zeros_like(u)
+        lib().orc_solid_residual(self.h, dptr(u), dptr(acc), dptr(b), dptr(R))
+        return R
+
+
+class Heat:
+    KIND = 1
+
+    def __init__(self, mesh, n_q=None, dt=0.1, coef=None):
+        self.mesh = mesh
+        n_q = mesh.p + 1 if n_q is None else n_q
+        self._coef = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+        self.h = lib().orc_heat_create(mesh.h, n_q, dt, None if self._coef is None else dptr(self._coef))
+        if not self.h:
+            raise ValueError("unsupported heat operator configuration")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_heat_destroy(self.h)
+            self.h = None
+
+    def vmult(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros_like(src)
+        lib().orc_heat_vmult(self.h, dptr(dst), dptr(src))
+        return dst
+
+    def vmult_noconstraints(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros_like(src)
+        lib().orc_heat_vmult_noconstraints(self.h, dptr(dst), dptr(src))
+        return dst
+
+    def vmult_bruteforce(self, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        dst = np.zeros_like(src)
+        lib().orc_heat_vmult_bruteforce(self.h, dptr(dst), dptr(src))
+        return dst
+
+    def diagonal(self):
+        d = np.zeros(self.mesh.n_dofs)
+        lib().orc_heat_diagonal(self.h, dptr(d))
+        return d
+
+    def load(self, f):
+        b = np.zeros(self.mesh.n_dofs)
+        lib().orc_heat_load(self.h, f, dptr(b))
+        return b
+
+
+def prolongate_add(cmesh, fmesh, c, f):
+    lib().orc_prolongate_add(cmesh.h, fmesh.h, dptr(c), dptr(f))
+
+
+def restrict_add(cmesh, fmesh, f, c):
+    lib().orc_restrict_add(cmesh.h, fmesh.h, dptr(f), dptr(c))
+
+
+def interpolate(cmesh, fmesh, f):
+    c = np.zeros(cmesh.n_dofs)
+    lib().orc_interpolate(cmesh.h, fmesh.h, dptr(np.ascontiguousarray(f)), dptr(c))
+    return c
+
+
+def mg_cg_solve(meshes, ops, b, cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0,
+                tol_rel=1e-8, max_iter=1000, x0=None):
+    L = len(meshes)
+    mh = (_vp * L)(*[m.h for m in meshes])
+    oh = (_vp * L)(*[o.h for o in ops])
+    prm = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, tol_rel, max_iter)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    rel = ctypes.c_double(0.0)
+    it = lib().orc_mg_cg_solve(ops[0].KIND, L, mh, oh, ctypes.byref(prm), dptr(np.ascontiguousarray(b)), dptr(x),
+                               ctypes.byref(rel))
+    return x, it, rel.value
+
+
+def cg_solve(op, b, precond=0, omega=1.0, tol_rel=1e-8, max_iter=10000):
+    x = np.zeros_like(b)
+    rel = ctypes.c_double(0.0)
+    it = lib().orc_cg_solve(op.KIND, op.h, precond, omega, dptr(np.ascontiguousarray(b)), dptr(x), tol_rel,
+                            max_iter, ctypes.byref(rel))
+    return x, it, rel.value
+
+
+def cheb_eigenvalues(op, n_it):
+    lmax, lmin = ctypes.c_double(0), ctypes.c_double(0)
+    lib().orc_cheb_eigenvalues(op.KIND, op.h, op.mesh.h, n_it, ctypes.byref(lmax), ctypes.byref(lmin))
+    return lmax.value, lmin.value
+
+
+def box_levels(lo, hi, coarse_n, n_levels, p, nc, face_mask=1, comp_mask=7):
+    """Nested structured boxes: level l has coarse_n * 2^l cells per direction."""
+    return [Mesh(lo, hi, [c << l for c in coarse_n], p, nc, face_mask, comp_mask) for l in range(n_levels)]

@@ -1,0 +1,180 @@
+"""Device operator parity: libmfgpu (through the C ABI) against the oracle on
+identical meshes and seeded inputs.  Bars: FP64 vmult relative l2 <= 1e-12,
+FP32 operators relative l2 <= 1e-5 (single-precision arithmetic, ~1e-7 per
+operation accumulated over the sum-factorisation sweeps)."""
+import numpy as np
+import pytest
+import torch
+
+import mfgpu
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if hasattr(a, "detach") else a
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def dev(x, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device="cuda", dtype=dtype)
+
+
+def solid_pair(lo, hi, n, p, form=1, face_mask=1, amp=5e-2, precision=mfgpu.FP64):
+    gm = mfgpu.BoxMesh(lo, hi, n, p, 3, face_mask, 7)
+    om = O.Mesh(lo, hi, n, p, 3, face_mask, 7)
+    u = om.u0(amp=amp, noise=1e-3)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, formulation=form, precision=precision)
+    go.cache(dev(u, go.dtype))
+    oo = O.Solid(om, formulation=form, caching=1)
+    oo.cache(u)
+    return gm, om, go, oo, u
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_solid_vmult_fp64_all_degrees(cuda, p):
+    print(f"solid vmult p={p}", flush=True)
+    gm, om, go, oo, u = solid_pair([0, 0, 0], [1.0, 0.6, 0.4], [3, 2, 2], p)
+    x = O.fill_uniform(1, om.n_dofs)
+    y = torch.empty(om.n_dofs, dtype=torch.float64, device="cuda")
+    go.vmult(y, dev(x))
+    ref = oo.vmult(x)
+    assert rel(y, ref) < TOL64
+    # constrained rows are exactly identity
+    con = om.constrained()
+    assert np.array_equal(y.cpu().numpy()[con], x[con])
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_solid_cache_matches_oracle(cuda, p):
+    gm, om, go, oo, u = solid_pair([0, 0, 0], [1.0, 0.6, 0.4], [2, 2, 1], p, amp=1e-1)
+    qd = go.qp_data().cpu().numpy()
+    ref = oo.tensor2_data()
+    for f in range(18):
+        scale = np.abs(ref[f]).max()
+        assert np.abs(qd[f] - ref[f]).max() <= 1e-12 * scale, f
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_solid_formulation0(cuda, p):
+    gm, om, go, oo, u = solid_pair([0, 0, 0], [1, 1, 1], [2, 2, 2], p, form=0)
+    x = O.fill_uniform(2, om.n_dofs)
+    y = torch.empty_like(dev(x))
+    go.vmult(y, dev(x))
+    assert rel(y, oo.vmult(x)) < TOL64
+
+
+def test_small_config_vmult_and_diagonal(cuda):
+    """8^3 Q2 unit cube, 14,739 dofs, u0 = 1e-3 sin sin sin + 1e-5 noise."""
+    gm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [8, 8, 8], 2, 3, 1, 7)
+    om = O.Mesh([0, 0, 0], [1, 1, 1], [8, 8, 8], 2, 3, 1, 7)
+    u = om.u0(amp=1e-3, noise=1e-5, seed=0)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
+    go.cache(dev(u))
+    oo = O.Solid(om, caching=1)
+    oo.cache(u)
+    x = O.fill_uniform(1, om.n_dofs)
+    y = torch.empty(om.n_dofs, dtype=torch.float64, device="cuda")
+    go.vmult(y, dev(x))
+    assert rel(y, oo.vmult(x)) < TOL64
+    go.compute_diagonal()
+    d = go.diagonal()
+    dref = oo.diagonal()
+    assert rel(d, dref) < TOL64
+    inv = go.get_matrix_diagonal_inverse().cpu().numpy()
+    assert np.abs(inv * dref - 1).max() < 1e-13
+    assert go.el(17, 17) == pytest.approx(dref[17], rel=1e-13)
+    with pytest.raises(mfgpu.MfgError):
+        go.el(1, 2)
+
+
+def test_vmult_add_tvmult_jacobi(cuda):
+    gm, om, go, oo, u = solid_pair([0, 0, 0], [1, 1, 1], [3, 3, 2], 2)
+    x = O.fill_uniform(3, om.n_dofs)
+    z = O.fill_uniform(4, om.n_dofs)
+    ref = oo.vmult(x)
+    y = dev(z)
+    go.vmult_add(y, dev(x))
+    assert rel(y, z + ref) < TOL64
+    y2 = torch.empty_like(y)
+    go.Tvmult(y2, dev(x))
+    assert rel(y2, ref) < TOL64
+    y3 = dev(z)
+    go.Tvmult_add(y3, dev(x))
+    assert rel(y3, z + ref) < TOL64
+    go.compute_diagonal()
+    j = torch.empty_like(y)
+    go.precondition_Jacobi(j, dev(x), 0.65)
+    assert rel(j, 0.65 * x / oo.diagonal()) < 1e-14
+    assert go.m() == go.n() == om.n_dofs
+    assert go.memory_consumption() >= 18 * 8 * om.n_cells * 27
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_heat_vmult_and_diagonal(cuda, p):
+    print(f"heat p={p}", flush=True)
+    lo, hi, n = [0, 0, 0], [1.0, 2.0, 0.5], [3, 2, 2]
+    gm = mfgpu.BoxMesh(lo, hi, n, p, 1, 63, 1)
+    om = O.Mesh(lo, hi, n, p, 1, 63, 1)
+    rng = np.random.default_rng(p)
+    coef = rng.uniform(0.5, 2.0, om.n_cells * (p + 1) ** 3)
+    go = mfgpu.LaplaceOperator(gm, 0.1, coef)
+    oo = O.Heat(om, dt=0.1, coef=coef)
+    x = O.fill_uniform(5, om.n_dofs)
+    y = torch.empty(om.n_dofs, dtype=torch.float64, device="cuda")
+    go.vmult(y, dev(x))
+    assert rel(y, oo.vmult(x)) < TOL64
+    go.compute_diagonal()
+    assert rel(go.diagonal(), oo.diagonal()) < TOL64
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_fp32_operators(cuda, p):
+    gm, om, go, oo, u = solid_pair([0, 0, 0], [1, 1, 1], [2, 2, 2], p, precision=mfgpu.FP32)
+    x = O.fill_uniform(1, om.n_dofs)
+    y = torch.empty(om.n_dofs, dtype=torch.float32, device="cuda")
+    go.vmult(y, dev(x, torch.float32))
+    assert rel(y.double(), oo.vmult(x)) < TOL32
+    go.compute_diagonal()
+    assert rel(go.diagonal().double(), oo.diagonal()) < TOL32
+    hm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [3, 3, 3], p, 1, 1, 1)
+    hg = mfgpu.LaplaceOperator(hm, 0.1, None, precision=mfgpu.FP32)
+    ho = O.Heat(O.Mesh([0, 0, 0], [1, 1, 1], [3, 3, 3], p, 1, 1, 1), dt=0.1)
+    xh = O.fill_uniform(2, hm.n_dofs)
+    yh = torch.empty(hm.n_dofs, dtype=torch.float32, device="cuda")
+    hg.vmult(yh, dev(xh, torch.float32))
+    assert rel(yh.double(), ho.vmult(xh)) < TOL32
+
+
+def test_ragged_and_single_cell_meshes(cuda):
+    for lo, hi, n, p, fm in [([0, 0, 0], [1, 1, 1], [1, 1, 1], 3, 0),
+                             ([0, 0, 0], [2, 1, 3], [5, 1, 3], 2, 63),
+                             ([0, 0, 0], [1, 1, 1], [7, 1, 1], 1, 1)]:
+        gm, om, go, oo, u = solid_pair(lo, hi, n, p, face_mask=fm)
+        x = O.fill_uniform(9, om.n_dofs)
+        y = torch.empty(om.n_dofs, dtype=torch.float64, device="cuda")
+        go.vmult(y, dev(x))
+        assert rel(y, oo.vmult(x)) < TOL64
+
+
+def test_error_paths(cuda):
+    gm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
+    y = torch.zeros(gm.n_dofs, dtype=torch.float64, device="cuda")
+    with pytest.raises(mfgpu.MfgError) as e:
+        go.vmult(y, y)  # before cache()
+    assert e.value.code == 5
+    # det F <= 0: a displacement that inverts cells
+    om = O.Mesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
+    X = om.node_coords()
+    u = np.zeros(om.n_dofs)
+    u[0::3] = -3.0 * X[:, 0]
+    with pytest.raises(mfgpu.MfgError) as e:
+        go.cache(dev(u))
+    assert e.value.code == 3 and "det_F" in str(e.value)
+    with pytest.raises(mfgpu.MfgError):
+        mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, caching="tensor4")

@@ -1,0 +1,231 @@
+"""Pins of the oracle's solid and heat operators.
+
+Every check here is independent of the oracle's own sum-factorised code path:
+dense quadrature assembly, closed forms, the finite-difference derivative of
+the residual, an energy computed in numpy, and linear elasticity at u = 0.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def unit_mesh(n, p, nc, face_mask=1):
+    return O.Mesh([0, 0, 0], [1, 1, 1], [n, n, n], p, nc, face_mask, 7)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_solid_sumfact_matches_dense_quadrature(p):
+    m = O.Mesh([0, 0, 0], [1.0, 0.7, 0.5], [2, 2, 1], p, 3)
+    u = m.u0(amp=5e-2, noise=1e-3)
+    s = O.Solid(m, caching=1)
+    s.cache(u)
+    x = O.fill_uniform(1, m.n_dofs)
+    y, yb = s.vmult(x), s.vmult_bruteforce(x)
+    assert np.linalg.norm(y - yb) / np.linalg.norm(yb) < 1e-13
+
+
+def test_cachings_define_the_same_operator():
+    m = unit_mesh(2, 2, 3)
+    u = m.u0(amp=1e-1, noise=1e-3)
+    x = O.fill_uniform(7, m.n_dofs)
+    ys = []
+    for caching in (0, 1, 2, 3):
+        s = O.Solid(m, caching=caching)
+        s.cache(u)
+        ys.append(s.vmult(x))
+    for y in ys[1:]:
+        assert np.linalg.norm(y - ys[0]) / np.linalg.norm(ys[0]) < 1e-13
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_tangent_is_symmetric(form):
+    m = unit_mesh(2, 2, 3)
+    s = O.Solid(m, formulation=form, caching=1)
+    s.cache(m.u0(amp=1e-1, noise=1e-3))
+    a, b = O.fill_uniform(3, m.n_dofs), O.fill_uniform(4, m.n_dofs)
+    ab, ba = a @ s.vmult(b), b @ s.vmult(a)
+    assert abs(ab - ba) < 1e-13 * abs(ab)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_tangent_is_derivative_of_residual(form):
+    m = unit_mesh(2, 2, 3)
+    u = m.u0(amp=1e-1, noise=1e-3)
+    s = O.Solid(m, formulation=form, caching=1)
+    s.cache(u)
+    v = O.fill_uniform(5, m.n_dofs)
+    con = m.constrained()
+    v[con] = 0.0
+    eps = 1e-7
+
+    def R(uu):  # Newmark acceleration a = alpha_1 (u - u_n) + const
+        return s.residual(uu, O.ALPHA1 * uu)
+
+    fd = -(R(u + eps * v) - R(u - eps * v)) / (2 * eps)
+    Av = s.vmult(v)
+    Av[con] = 0.0
+    assert np.linalg.norm(fd - Av) / np.linalg.norm(Av) < 1e-8
+
+
+def _numpy_energy(m, u, mu, lam):
+    """int Psi dX for Psi = mu/2 (tr C - 3 - 2 lnJ) + lam (lnJ)^2, evaluated in
+    numpy with its own tensor-product quadrature (one box cell layout)."""
+    p, N = m.p, m.p + 1
+    z = O.gll_points(p)
+    xq, wq = O.gauss_rule(N)
+    S, D = O.lagrange(z, xq)
+    nodes = m.cell_nodes().reshape(m.n_cells, N, N, N)  # [cell][c][b][a]
+    verts = m.cell_vertices().reshape(m.n_cells, 8, 3)
+    E = 0.0
+    for c in range(m.n_cells):
+        h = verts[c, 7] - verts[c, 0]
+        U = np.stack([u[3 * nodes[c] + d] for d in range(3)])  # [d][kc][jb][ia]
+        # grad_xi u: [d][f] at (q2,q1,q0)
+        g0 = np.einsum("ia,jb,kc,dcba->dkji", D, S, S, U)
+        g1 = np.einsum("ia,jb,kc,dcba->dkji", S, D, S, U)
+        g2 = np.einsum("ia,jb,kc,dcba->dkji", S, S, D, U)
+        G = np.stack([g0 / h[0], g1 / h[1], g2 / h[2]], axis=1)  # [d][e]
+        F = G + np.eye(3)[:, :, None, None, None]
+        F = np.moveaxis(F, (0, 1), (-2, -1))
+        J = np.linalg.det(F)
+        trC = np.einsum("...ij,...ij->...", F, F)
+        psi = mu / 2 * (trC - 3 - 2 * np.log(J)) + lam * np.log(J) ** 2
+        W = np.einsum("k,j,i->kji", wq, wq, wq) * np.prod(h)
+        E += np.sum(psi * W)
+    return E
+
+
+def test_residual_is_minus_energy_gradient():
+    m = O.Mesh([0, 0, 0], [1.0, 0.8, 0.6], [1, 1, 1], 2, 3, face_mask=0)
+    u = m.u0(amp=1e-1, noise=1e-2)
+    s = O.Solid(m, caching=1)
+    lam = 2 * O.MU * O.NU / (1 - 2 * O.NU)
+    R = s.residual(u, np.zeros_like(u))
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        v = rng.uniform(-1, 1, m.n_dofs)
+        eps = 1e-6
+        dE = (_numpy_energy(m, u + eps * v, O.MU, lam) - _numpy_energy(m, u - eps * v, O.MU, lam)) / (2 * eps)
+        assert abs(dE + R @ v) < 1e-6 * abs(dE)
+
+
+def _linear_elasticity_dense(m, mu, lam):
+    """Dense linear elasticity stiffness with Lame pair (mu, lam), numpy."""
+    p, N = m.p, m.p + 1
+    z = O.gll_points(p)
+    xq, wq = O.gauss_rule(N)
+    S, D = O.lagrange(z, xq)
+    K = np.zeros((m.n_dofs, m.n_dofs))
+    nodes = m.cell_nodes()
+    verts = m.cell_vertices().reshape(m.n_cells, 8, 3)
+    for c in range(m.n_cells):
+        h = verts[c, 7] - verts[c, 0]
+        cn = nodes[c * N ** 3:(c + 1) * N ** 3]
+        for q2 in range(N):
+            for q1 in range(N):
+                for q0 in range(N):
+                    w = wq[q0] * wq[q1] * wq[q2] * np.prod(h)
+                    grads = np.zeros((N ** 3, 3))
+                    for kc in range(N):
+                        for jb in range(N):
+                            for ia in range(N):
+                                a = ia + N * (jb + N * kc)
+                                grads[a] = [D[q0, ia] * S[q1, jb] * S[q2, kc] / h[0],
+                                            S[q0, ia] * D[q1, jb] * S[q2, kc] / h[1],
+                                            S[q0, ia] * S[q1, jb] * D[q2, kc] / h[2]]
+                    # B matrix rows: strain components
+                    for a in range(N ** 3):
+                        for b in range(N ** 3):
+                            ga, gb = grads[a], grads[b]
+                            # k_ab[d][e] = mu (ga.gb delta_de + gb_d ga_e) + lam ga_d gb_e
+                            kab = mu * (ga @ gb * np.eye(3) + np.outer(gb, ga)) + lam * np.outer(ga, gb)
+                            for d in range(3):
+                                for e in range(3):
+                                    K[3 * cn[a] + d, 3 * cn[b] + e] += w * kab[d, e]
+    return K
+
+
+@pytest.mark.parametrize("form,lame_factor", [(1, 2.0), (0, 1.0)])
+def test_u0_reduces_to_linear_elasticity(form, lame_factor):
+    m = O.Mesh([0, 0, 0], [1.0, 0.5, 0.5], [2, 1, 1], 1, 3, face_mask=0)
+    s = O.Solid(m, formulation=form, caching=1, alpha1=0.0)
+    s.cache(np.zeros(m.n_dofs))
+    lam = 2 * O.MU * O.NU / (1 - 2 * O.NU)
+    K = _linear_elasticity_dense(m, O.MU, lame_factor * lam)
+    A = np.stack([s.vmult(e) for e in np.eye(m.n_dofs)], axis=1)
+    assert np.abs(A - K).max() < 1e-10 * np.abs(K).max()
+
+
+def test_mass_closed_form_any_linearisation_point():
+    # constant fields have zero strain: e_d^T A e_d = rho alpha_1 |Omega|
+    m = O.Mesh([0, 0, 0], [4.0, 1.0, 1.0], [4, 1, 1], 2, 3, face_mask=0)
+    s = O.Solid(m, caching=1)
+    s.cache(m.u0(amp=1e-1, noise=1e-3))
+    for d in range(3):
+        e = np.zeros(m.n_dofs)
+        e[d::3] = 1.0
+        assert abs(e @ s.vmult(e) - O.RHO * O.ALPHA1 * 4.0) < 1e-11 * O.RHO * O.ALPHA1 * 4.0
+
+
+def test_diagonal_matches_unit_probes():
+    m = unit_mesh(2, 2, 3)
+    s = O.Solid(m, caching=1)
+    s.cache(m.u0(amp=1e-1, noise=1e-3))
+    d = s.diagonal()
+    con = set(m.constrained().tolist())
+    for i in range(0, m.n_dofs, 7):
+        e = np.zeros(m.n_dofs)
+        e[i] = 1.0
+        ref = 1.0 if i in con else s.vmult_bruteforce(e)[i]
+        assert abs(d[i] - ref) < 1e-12 * abs(ref)
+
+
+def test_constrained_rows_are_identity():
+    m = unit_mesh(2, 3, 3)
+    s = O.Solid(m, caching=1)
+    s.cache(m.u0())
+    x = O.fill_uniform(1, m.n_dofs)
+    y = s.vmult(x)
+    con = m.constrained()
+    assert np.array_equal(y[con], x[con])
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_heat_sumfact_matches_dense_and_mass(p):
+    m = O.Mesh([0, 0, 0], [1.0, 2.0, 0.5], [2, 1, 2], p, 1, face_mask=0)
+    rng = np.random.default_rng(p)
+    coef = rng.uniform(0.5, 2.0, m.n_cells * (p + 1) ** 3)
+    h = O.Heat(m, dt=0.1, coef=coef)
+    x = O.fill_uniform(1, m.n_dofs)
+    assert np.linalg.norm(h.vmult(x) - h.vmult_bruteforce(x)) < 1e-13 * np.linalg.norm(h.vmult_bruteforce(x))
+    one = np.ones(m.n_dofs)
+    assert abs(one @ h.vmult(one) - 1.0) < 1e-13  # |Omega| = 1
+
+
+def test_heat_manufactured_quadratic_solution_is_exact():
+    """u = 1 + x^2 + a y^2 + g z^2 + b t solves u_t - lap u = b - 2 - 2a - 2g
+    and is reproduced exactly by Q2 + backward Euler (partitioned_heat.h:22-81)."""
+    a, b, g, dt = 3.0, 1.3, 0.7, 0.1
+    m = O.Mesh([0, 0, 0], [1, 1, 1], [3, 3, 3], 2, 1, face_mask=63, comp_mask=1)
+    mf = O.Mesh([0, 0, 0], [1, 1, 1], [3, 3, 3], 2, 1, face_mask=0, comp_mask=1)
+    X = m.node_coords()
+
+    def exact(t):
+        return 1 + X[:, 0] ** 2 + a * X[:, 1] ** 2 + g * X[:, 2] ** 2 + b * t
+
+    A = O.Heat(m, dt=dt)
+    A_full = O.Heat(mf, dt=dt)
+    M = O.Heat(mf, dt=0.0)
+    con = m.constrained()
+    u = exact(0.0)
+    for step in range(1, 3):
+        t = step * dt
+        ud = np.zeros(m.n_dofs)
+        ud[con] = exact(t)[con]
+        rhs = M.vmult_noconstraints(u) + dt * A.load(b - 2 - 2 * a - 2 * g) - A_full.vmult_noconstraints(ud)
+        rhs[con] = 0.0
+        w, it, rel = O.cg_solve(A, rhs, precond=1, tol_rel=1e-14, max_iter=2000)
+        assert it > 0
+        u = w + ud
+        assert np.abs(u - exact(t)).max() < 1e-11
