@@ -141,6 +141,11 @@ size_t mfg_operator_memory_consumption(mfg_operator op);
  * Jinv (9), JxW, c.  Copies n_fields*n_qp values to the device pointer out;
  * n_fields may be queried with out == NULL. */
 int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields);
+/* Number of cell colours of the device layout.  Cells are stored
+ * colour-major; cells of one colour share no node, so each colour is one
+ * launch whose scatter needs no atomics (deterministic results).  Replaces the
+ * role of MatrixFree's cell-batch partition (mf_elasticity.h:806-859). */
+int32_t mfg_operator_n_colours(mfg_operator op);
 
 /* Kernel timing (TimerOutput-style sections of the reference,
  * mf_elasticity.h:427-432): when enabled, every vmult/vmult_add records HIP

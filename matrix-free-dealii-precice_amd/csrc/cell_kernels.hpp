@@ -342,11 +342,14 @@ struct QHeat {
 };
 
 // ------------------------------------------------------ apply kernel ---
+// One launch covers the cells [cell_begin, cell_end) of one colour: no two of
+// them share a node, so the scatter is a plain read-modify-write and the
+// result is bit-reproducible (cells are stored colour-major).
 // fix_mode: 0 no constrained-row handling, 1 dst[c] = src[c], 2 dst[c] += src[c]
 template <typename Number>
 struct ApplyArgs {
   const uint32_t *cell_nodes;
-  int n_cells;
+  int cell_begin, cell_end;
   const Number *src;
   Number *dst;
   const Number *shape;   // N x N
@@ -387,8 +390,8 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
   T.j = (pt / N) % N;
   T.k = pt / (N * N);
   T.writer = !pad;
-  const int cell = xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
-  const bool active = !pad && cell < a.n_cells;
+  const int cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool active = !pad && cell < a.cell_end;
   const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
 
   uint32_t entry = active ? a.cell_nodes[qidx] : 0u;
@@ -410,7 +413,7 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      if (!((entry >> (kConstraintShift + c)) & 1u)) atomic_add(&a.dst[node * NC + c], out[c]);
+      if (!((entry >> (kConstraintShift + c)) & 1u)) a.dst[node * NC + c] += out[c];
   }
 }
 

@@ -36,29 +36,33 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
   constexpr int CPB = DefaultCPB<P>::value;
   ApplyArgs<Number> a;
   a.cell_nodes = op.cell_nodes.as<uint32_t>();
-  a.n_cells = op.n_cells;
   a.src = (const Number *)src;
   a.dst = (Number *)dst;
   a.shape = op.tab_shape.as<Number>();
   a.colloc = op.tab_colloc.as<Number>();
   a.constrained = op.constrained.as<int32_t>();
   a.n_constrained = op.n_constrained;
-  a.fix_mode = fix_mode;
-  const int grid = blocks_for(op.n_cells, CPB);
   constexpr int NT = block_threads<P, CPB>();
-  if (op.kind == 0) {
-    if (op.sp.formulation == 1) {
-      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
-      apply_cells<P, CPB, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+  for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
+    a.cell_begin = op.colour_start[col];
+    a.cell_end = op.colour_start[col + 1];
+    a.fix_mode = col == 0 ? fix_mode : 0;
+    const int grid = blocks_for(a.cell_end - a.cell_begin, CPB);
+    if (grid == 0) continue;
+    if (op.kind == 0) {
+      if (op.sp.formulation == 1) {
+        QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+        apply_cells<P, CPB, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      } else {
+        QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
+        apply_cells<P, CPB, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      }
     } else {
-      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
-      apply_cells<P, CPB, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+      apply_cells<P, CPB, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
     }
-  } else {
-    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
-    apply_cells<P, CPB, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+    MFG_HIP(hipGetLastError());
   }
-  MFG_HIP(hipGetLastError());
 }
 
 template <int P, typename Number>
@@ -89,23 +93,28 @@ template <int P, typename Number>
 static void launch_diagonal(OperatorData &op) {
   DiagArgs<Number> a;
   a.cell_nodes = op.cell_nodes.as<uint32_t>();
-  a.n_cells = op.n_cells;
   a.sq = op.tab_sq.as<Number>();
   a.diag = op.diag.as<Number>();
   constexpr int NT = ((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64;
-  if (op.kind == 0) {
-    if (op.sp.formulation == 1) {
-      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
-      diagonal_cells<P, Number, QSolidT2F1<Number>><<<op.n_cells, NT, 0, op.stream>>>(a, q);
+  for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
+    a.cell_begin = op.colour_start[col];
+    a.cell_end = op.colour_start[col + 1];
+    const int grid = a.cell_end - a.cell_begin;
+    if (grid == 0) continue;
+    if (op.kind == 0) {
+      if (op.sp.formulation == 1) {
+        QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+        diagonal_cells<P, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      } else {
+        QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
+        diagonal_cells<P, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      }
     } else {
-      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
-      diagonal_cells<P, Number, QSolidT2F0<Number>><<<op.n_cells, NT, 0, op.stream>>>(a, q);
+      QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+      diagonal_cells<P, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
     }
-  } else {
-    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
-    diagonal_cells<P, Number, QHeat<Number>><<<op.n_cells, NT, 0, op.stream>>>(a, q);
+    MFG_HIP(hipGetLastError());
   }
-  MFG_HIP(hipGetLastError());
 }
 
 #define MFG_DISPATCH_DEGREE(p, FN, ...)                                                  \
@@ -259,6 +268,44 @@ static void upload_tables(OperatorData &op, const Tables1D &t) {
   upload(op.tab_weights, t.weight.data(), t.weight.size(), op.stream);
 }
 
+// Greedy colouring of the cells in their given order such that no two cells
+// of one colour share a node; on a lexicographic hex grid this yields the
+// 8 parity colours.  Fills perm (colour-major order -> given cell index) and
+// colour_start.
+void OperatorData::colour_cells(const mfg_level_desc &d, int N3) {
+  std::vector<int64_t> off((size_t)n_nodes + 1, 0);
+  const int64_t ne = (int64_t)n_cells * N3;
+  for (int64_t x = 0; x < ne; ++x) off[d.cell_nodes[x] + 1]++;
+  for (int64_t i = 0; i < n_nodes; ++i) off[i + 1] += off[i];
+  std::vector<int32_t> cells((size_t)ne);
+  {
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    for (int64_t x = 0; x < ne; ++x) cells[pos[d.cell_nodes[x]]++] = (int32_t)(x / N3);
+  }
+  std::vector<int8_t> colour(n_cells, -1);
+  int n_col = 0;
+  for (int c = 0; c < n_cells; ++c) {
+    uint64_t used = 0;
+    for (int x = 0; x < N3; ++x) {
+      const int32_t node = d.cell_nodes[(int64_t)c * N3 + x];
+      for (int64_t k = off[node]; k < off[node + 1]; ++k) {
+        const int8_t cc = colour[cells[k]];
+        if (cc >= 0) used |= (uint64_t)1 << cc;
+      }
+    }
+    const int cc = __builtin_ctzll(~used);
+    MFG_REQUIRE(cc < 64, MFG_ERR_ARGUMENT, "cell colouring needs more than 64 colours");
+    colour[c] = (int8_t)cc;
+    n_col = std::max(n_col, cc + 1);
+  }
+  colour_start.assign(n_col + 1, 0);
+  for (int c = 0; c < n_cells; ++c) colour_start[colour[c] + 1]++;
+  for (int k = 0; k < n_col; ++k) colour_start[k + 1] += colour_start[k];
+  perm.assign(n_cells, 0);
+  std::vector<int> fillp(colour_start.begin(), colour_start.end() - 1);
+  for (int c = 0; c < n_cells; ++c) perm[fillp[colour[c]]++] = c;
+}
+
 void OperatorData::setup_layout(const mfg_level_desc &d) {
   MFG_REQUIRE(d.degree >= 1 && d.degree <= kMaxDegree, MFG_ERR_ARGUMENT, "degree must be in 1..6");
   MFG_REQUIRE(d.n_q_1d == d.degree + 1, MFG_ERR_ARGUMENT, "n_q_1d must equal degree+1");
@@ -281,16 +328,27 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
     MFG_REQUIRE(dof >= 0 && dof < n_dofs, MFG_ERR_ARGUMENT, "constrained dof out of range");
     cmask[dof / nc] |= (uint8_t)(1u << (dof % nc));
   }
-  std::vector<uint32_t> map((size_t)nqp);
   for (int64_t x = 0; x < nqp; ++x) {
     const int32_t node = d.cell_nodes[x];
     MFG_REQUIRE(node >= 0 && node < n_nodes, MFG_ERR_ARGUMENT, "cell node index out of range");
-    map[x] = (uint32_t)node | ((uint32_t)cmask[node] << kNodeBits);
+  }
+  colour_cells(d, N3);
+  // device layout in colour-major cell order
+  std::vector<uint32_t> map((size_t)nqp);
+  std::vector<double> vert((size_t)n_cells * 24);
+  for (int c = 0; c < n_cells; ++c) {
+    const int64_t o = perm[c];
+    for (int x = 0; x < N3; ++x) {
+      const int32_t node = d.cell_nodes[o * N3 + x];
+      map[(size_t)c * N3 + x] = (uint32_t)node | ((uint32_t)cmask[node] << kNodeBits);
+    }
+    for (int x = 0; x < 24; ++x) vert[(size_t)c * 24 + x] = d.cell_vertices[o * 24 + x];
   }
   upload(cell_nodes, map.data(), map.size(), stream);
   upload(constrained, d.constrained_dofs, (size_t)n_constrained, stream);
   host_constrained.assign(d.constrained_dofs, d.constrained_dofs + n_constrained);
-  upload(vertices, d.cell_vertices, (size_t)n_cells * 24, stream);
+  upload(vertices, vert.data(), vert.size(), stream);
+  upload(perm_dev, perm.data(), perm.size(), stream);
   flag.allocate(sizeof(int));
   const Tables1D t = make_tables(p);
   if (precision == MFG_FP64) upload_tables<double>(*this, t);
@@ -306,6 +364,7 @@ static void heat_geometry(OperatorData &op, const mfg_level_desc &d, const doubl
   const int64_t n = op.nqp;
   for (int c = 0; c < op.n_cells; ++c)
     for (int q = 0; q < N3; ++q) {
+      const int64_t orig = op.perm[c];  // colour-major cell c is input cell orig
       const int i = q % N, j = (q / N) % N, k = q / (N * N);
       const double x[3] = {t.points[i], t.points[j], t.points[k]};
       double J[3][3] = {};
@@ -318,7 +377,7 @@ static void heat_geometry(OperatorData &op, const mfg_level_desc &d, const doubl
         }
         const double g[3] = {dd[0] * nn[1] * nn[2], nn[0] * dd[1] * nn[2], nn[0] * nn[1] * dd[2]};
         for (int e = 0; e < 3; ++e)
-          for (int f = 0; f < 3; ++f) J[e][f] += d.cell_vertices[(size_t)c * 24 + v * 3 + e] * g[f];
+          for (int f = 0; f < 3; ++f) J[e][f] += d.cell_vertices[orig * 24 + v * 3 + e] * g[f];
       }
       const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -332,7 +391,7 @@ static void heat_geometry(OperatorData &op, const mfg_level_desc &d, const doubl
       const int64_t o = (int64_t)c * N3 + q;
       for (int f = 0; f < 9; ++f) h[f * n + o] = (Number)R[f];
       h[9 * n + o] = (Number)(det * t.weight[i] * t.weight[j] * t.weight[k]);
-      h[10 * n + o] = (Number)(coef ? coef[o] : 1.0);
+      h[10 * n + o] = (Number)(coef ? coef[orig * N3 + q] : 1.0);
     }
   upload(op.qd, h.data(), h.size(), op.stream);
 }
@@ -523,9 +582,20 @@ int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields) {
     if (n_fields) *n_fields = op->n_fields;
     if (out) {
       MFG_REQUIRE(op->ready, MFG_ERR_STATE, "no quadrature data yet");
-      MFG_HIP(hipMemcpyAsync(out, op->qd.ptr, op->qd.bytes, hipMemcpyDeviceToDevice, op->stream));
+      const int n3 = (op->p + 1) * (op->p + 1) * (op->p + 1);
+      const int threads = 256;
+      const int blocks = std::min(blocks_for(op->nqp * op->n_fields, threads), 65536);
+      if (op->precision == MFG_FP64)
+        unpermute_fields_kernel<double><<<blocks, threads, 0, op->stream>>>(
+            op->qd.as<double>(), (double *)out, op->perm_dev.as<int32_t>(), op->n_cells, n3, op->n_fields);
+      else
+        unpermute_fields_kernel<float><<<blocks, threads, 0, op->stream>>>(
+            op->qd.as<float>(), (float *)out, op->perm_dev.as<int32_t>(), op->n_cells, n3, op->n_fields);
+      MFG_HIP(hipGetLastError());
     }
   });
 }
+
+int32_t mfg_operator_n_colours(mfg_operator op) { return op ? (int32_t)op->colour_start.size() - 1 : -1; }
 
 }  // extern "C"

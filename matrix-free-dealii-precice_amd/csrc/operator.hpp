@@ -22,6 +22,12 @@ struct OperatorData {
   DeviceBuffer diag, inv_diag;
   DeviceBuffer flag;         // int error flag
   std::vector<int32_t> host_constrained;
+  // cells are stored colour-major: perm[c] = input index of stored cell c,
+  // colour k occupies stored cells [colour_start[k], colour_start[k+1])
+  std::vector<int32_t> perm;
+  std::vector<int> colour_start;
+  DeviceBuffer perm_dev;
+  void colour_cells(const mfg_level_desc &d, int N3);
   // optional cell-kernel timing
   bool timing = false;
   mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -174,7 +174,7 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
 template <typename Number>
 struct DiagArgs {
   const uint32_t *cell_nodes;
-  int n_cells;
+  int cell_begin, cell_end;  // one colour: cells share no node
   const Number *sq;  // 3 x N x N: SS, SD, DD
   Number *diag;
 };
@@ -195,8 +195,8 @@ __global__ void __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) 
   T.j = (pt / N) % N;
   T.k = pt / (N * N);
   T.writer = !pad;
-  const int cell = blockIdx.x;
-  const bool active = !pad && cell < a.n_cells;
+  const int cell = a.cell_begin + blockIdx.x;
+  const bool active = !pad && cell < a.cell_end;
   const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
   Number coef[NTERM];
   {
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) 
     Number s = 0;
 #pragma unroll
     for (int x = 0; x < 7; ++x) s += T.template line_t<0>(s_sq + mat[x][0] * N * N, T.i, d * 7 + x);
-    if (!((entry >> (kConstraintShift + d)) & 1u)) atomic_add(&a.diag[node * NC + d], s);
+    if (!((entry >> (kConstraintShift + d)) & 1u)) a.diag[node * NC + d] += s;
   }
 }
 

@@ -70,6 +70,7 @@ SIGNATURES = {
     "mfg_operator_precondition_jacobi": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double]),
     "mfg_operator_memory_consumption": (ctypes.c_size_t, [_vp]),
     "mfg_operator_qp_data": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32)]),
+    "mfg_operator_n_colours": (ctypes.c_int32, [_vp]),
     "mfg_operator_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mfg_operator_timing": (ctypes.c_int, [_vp, _c_dp, ctypes.POINTER(ctypes.c_int64)]),
     "mfg_fill_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
@@ -134,6 +135,7 @@ class BoxMesh:
                                          self._n.ctypes.data_as(_c_i32p), degree, n_components, face_mask,
                                          comp_mask, ctypes.byref(h)))
         self.h = h
+        self._destroy = lib().mfg_box_mesh_destroy
         self.desc = LevelDesc()
         check(lib().mfg_box_mesh_describe(self.h, ctypes.byref(self.desc)))
         self.degree, self.n_components = degree, n_components
@@ -145,7 +147,7 @@ class BoxMesh:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().mfg_box_mesh_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def cell_nodes(self):
@@ -182,10 +184,11 @@ class _Operator:
         self.h = handle
         self.precision = precision
         self._keep = keep
+        self._destroy = lib().mfg_operator_destroy
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().mfg_operator_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     @property
@@ -238,6 +241,9 @@ class _Operator:
 
     def memory_consumption(self):
         return lib().mfg_operator_memory_consumption(self.h)
+
+    def n_colours(self):
+        return lib().mfg_operator_n_colours(self.h)
 
     def enable_timing(self, enable=True):
         check(lib().mfg_operator_enable_timing(self.h, 1 if enable else 0))

@@ -161,6 +161,18 @@ def test_ragged_and_single_cell_meshes(cuda):
         assert rel(y, oo.vmult(x)) < TOL64
 
 
+def test_colouring_and_bit_reproducibility(cuda):
+    gm, om, go, oo, u = solid_pair([0, 0, 0], [1, 1, 1], [5, 4, 3], 2)
+    assert go.n_colours() == 8
+    x = dev(O.fill_uniform(3, om.n_dofs))
+    y1, y2 = torch.empty_like(x), torch.empty_like(x)
+    go.vmult(y1, x)
+    go.vmult(y2, x)
+    assert torch.equal(y1, y2)
+    one = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [1, 1, 1], 2, 3, 1, 7)
+    assert mfgpu.NeoHookOperator(one, O.MU, O.NU, O.RHO, O.ALPHA1).n_colours() == 1
+
+
 def test_error_paths(cuda):
     gm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
     go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
