@@ -83,6 +83,8 @@ int mfg_box_mesh_destroy(mfg_mesh mesh);
 int mfg_box_mesh_describe(mfg_mesh mesh, mfg_level_desc *desc);
 /* Node coordinates, n_nodes*3 doubles, host memory. */
 int mfg_box_mesh_node_coords(mfg_mesh mesh, double *out);
+/* Cells per direction. */
+int mfg_box_mesh_cells(mfg_mesh mesh, int32_t n_cells[3]);
 
 /* ------------------------------------------------------------ operators --- */
 typedef struct mfg_operator_t *mfg_operator;
@@ -154,6 +156,74 @@ int32_t mfg_operator_n_colours(mfg_operator op);
  * launch count since the last query, and resets both. */
 int mfg_operator_enable_timing(mfg_operator op, int32_t enable);
 int mfg_operator_timing(mfg_operator op, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------------ multigrid transfer --- */
+/* Transfer between nested levels (MGTransferMatrixFree, used at
+ * mf_elasticity.h:863-877, 920-922; heat_transfer.h:761-762).  children holds,
+ * for every coarse cell (input order), its 8 fine cells (input order) with
+ * child index dx + 2 dy + 4 dz.  Operators may differ in precision.  */
+typedef struct mfg_transfer_t *mfg_transfer;
+int mfg_transfer_create(mfg_operator coarse, mfg_operator fine, const int32_t *children, mfg_transfer *out);
+int mfg_transfer_destroy(mfg_transfer t);
+/* children table of a box mesh and its once-refined box (n_fine = 2 n_coarse) */
+int mfg_box_mesh_children(mfg_mesh coarse, mfg_mesh fine, int32_t *children);
+/* fine += P coarse (prolongate_and_add); coarse constrained read as zero,
+ * fine constrained dofs untouched.  Vectors in the operators' precisions. */
+int mfg_transfer_prolongate_add(mfg_transfer t, void *fine, const void *coarse);
+/* coarse += P^T fine (restrict_and_add); fine constrained read as zero,
+ * coarse constrained untouched. */
+int mfg_transfer_restrict_add(mfg_transfer t, void *coarse, const void *fine);
+/* coarse = nodal interpolant of the fine function (interpolate_to_mg). */
+int mfg_transfer_interpolate(mfg_transfer t, void *coarse, const void *fine);
+
+/* ------------------------------------------------ multigrid preconditioner --- */
+/* Geometric V-cycle with Chebyshev smoothing (Multigrid + PreconditionMG +
+ * mg::SmootherRelaxation<PreconditionChebyshev>, mf_elasticity.h:1140-1279,
+ * heat_transfer.h:757-812).  Levels > 0: Chebyshev of `cheb_degree` with the
+ * inverse diagonal, interval [1.2 lmax/smoothing_range, 1.2 lmax], lmax from
+ * eig_cg_iterations Jacobi-preconditioned CG steps.  Level 0: coarse_kind 0 =
+ * Chebyshev as a solver (range coarse_range, degree from the estimate, m()
+ * eigenvalue iterations); 1 = CG to relative coarse_tolerance (absolute
+ * 1e-10, max(m,100) steps) preconditioned by the level-0 Chebyshev. */
+typedef struct mfg_mg_params {
+  int32_t cheb_degree;
+  double smoothing_range;
+  int32_t eig_cg_iterations;
+  int32_t coarse_kind;
+  double coarse_range;
+  double coarse_tolerance;
+} mfg_mg_params;
+typedef struct mfg_mg_t *mfg_mg;
+/* ops[0..n_levels-1] coarse to fine; transfers[l-1] connects level l-1 and l.
+ * The hierarchy references (does not own) operators and transfers. */
+int mfg_mg_create(int32_t n_levels, const mfg_operator *ops, const mfg_transfer *transfers,
+                  const mfg_mg_params *params, mfg_mg *out);
+int mfg_mg_destroy(mfg_mg mg);
+/* (Re-)initialise after the level operators changed (cache()): diagonals,
+ * eigenvalue estimates, Chebyshev coefficients (setup_gmg, :1140-1279). */
+int mfg_mg_setup(mfg_mg mg);
+/* dst = V-cycle(src) (PreconditionMG::vmult); vec_precision of dst/src. */
+int mfg_mg_vmult(mfg_mg mg, void *dst, const void *src, int32_t vec_precision);
+/* Chebyshev data of a level: lmax, lmin estimates and polynomial degree. */
+int mfg_mg_level_info(mfg_mg mg, int32_t level, double *lmax, double *lmin, int32_t *degree);
+
+/* ------------------------------------------------------------------- CG --- */
+/* SolverCG (Solid::solve_linear_system, mf_elasticity.h:1981-2070): solve
+ * A x = b from the initial x to ||r|| <= tol_rel*||b||.  preconditioner:
+ * MFG_PRECOND_NONE, MFG_PRECOND_JACOBI (omega * D^-1, PreconditionJacobi) or
+ * MFG_PRECOND_MG (mg must be set).  Vectors in A's precision.  Returns
+ * MFG_ERR_NOCONV when max_iter is reached (info still filled). */
+#define MFG_PRECOND_NONE 0
+#define MFG_PRECOND_JACOBI 1
+#define MFG_PRECOND_MG 2
+typedef struct mfg_solve_info {
+  int32_t iterations;
+  double initial_residual;
+  double final_residual;
+  double condition_estimate;  /* from the Lanczos coefficients of the solve */
+} mfg_solve_info;
+int mfg_cg_solve(mfg_operator A, int32_t preconditioner, mfg_mg mg, double omega, const void *b, void *x,
+                 double tol_rel, int32_t max_iter, mfg_solve_info *info);
 
 /* ------------------------------------------------------ synthetic inputs --- */
 /* dst[i] = 2*U(seed,i) - 1 with U = (splitmix64(seed,i) >> 11) * 2^-53, on the

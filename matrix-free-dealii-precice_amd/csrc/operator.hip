@@ -349,6 +349,8 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
   host_constrained.assign(d.constrained_dofs, d.constrained_dofs + n_constrained);
   upload(vertices, vert.data(), vert.size(), stream);
   upload(perm_dev, perm.data(), perm.size(), stream);
+  host_cell_nodes.assign(d.cell_nodes, d.cell_nodes + nqp);
+  host_cmask = std::move(cmask);
   flag.allocate(sizeof(int));
   const Tables1D t = make_tables(p);
   if (precision == MFG_FP64) upload_tables<double>(*this, t);

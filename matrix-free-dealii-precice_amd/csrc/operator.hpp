@@ -22,6 +22,8 @@ struct OperatorData {
   DeviceBuffer diag, inv_diag;
   DeviceBuffer flag;         // int error flag
   std::vector<int32_t> host_constrained;
+  std::vector<int32_t> host_cell_nodes;  // input cell order (used to build transfers)
+  std::vector<uint8_t> host_cmask;       // per node: constrained components
   // cells are stored colour-major: perm[c] = input index of stored cell c,
   // colour k occupies stored cells [colour_start[k], colour_start[k+1])
   std::vector<int32_t> perm;

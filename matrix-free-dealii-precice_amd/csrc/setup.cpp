@@ -256,6 +256,13 @@ int mfg_box_mesh_fill_u0(mfg_mesh m, double amp, double noise, uint64_t seed, do
   });
 }
 
+int mfg_box_mesh_cells(mfg_mesh m, int32_t n[3]) {
+  return guarded([&] {
+    MFG_REQUIRE(m && n, MFG_ERR_ARGUMENT, "null argument");
+    for (int d = 0; d < 3; ++d) n[d] = m->n[d];
+  });
+}
+
 int mfg_box_mesh_node_coords(mfg_mesh m, double *out) {
   return guarded([&] {
     MFG_REQUIRE(m && out, MFG_ERR_ARGUMENT, "null argument");

@@ -1,0 +1,77 @@
+// Host-side objects of the multigrid preconditioner and CG solver.
+#pragma once
+#include "common.hpp"
+#include "operator.hpp"
+
+namespace mfg {
+
+// Deterministic device reductions with a host read-back.
+struct Reducer {
+  DeviceBuffer partials, result;
+  Reducer();
+  double dot(int prec, const void *a, const void *b, int64_t n, hipStream_t s);
+  // x += alpha p; r -= alpha h; returns r.r
+  double cg_update(int prec, void *x, void *r, const void *p, const void *h, double alpha, int64_t n, hipStream_t s);
+
+ private:
+  template <typename T> double dot_t(const void *a, const void *b, int64_t n, hipStream_t s);
+  template <typename T>
+  double cg_update_t(void *x, void *r, const void *p, const void *h, double alpha, int64_t n, hipStream_t s);
+};
+
+void vec_fill(int prec, void *x, int64_t n, double v, hipStream_t s);
+void vec_convert(int prec_dst, void *dst, int prec_src, const void *src, int64_t n, hipStream_t s);
+void vec_xpby(int prec, void *p, const void *d, double beta, int64_t n, hipStream_t s);
+void vec_residual(int prec, void *t, const void *b, int64_t n, hipStream_t s);
+void vec_add_scaled(int prec, void *y, const void *a, double sc, const void *b, int64_t n, hipStream_t s);
+void vec_jacobi(int prec, void *dst, const void *invD, const void *src, double omega, int64_t n, hipStream_t s);
+
+struct TransferData {
+  OperatorData *coarse = nullptr, *fine = nullptr;
+  int p = 1, nc = 1;
+  DeviceBuffer lattice;  // (2p+1)^3 entries per coarse cell (stored order)
+  DeviceBuffer E, I;     // embedding (2p+1)x(p+1), interpolation (p+1)x(2p+1)
+  void build(OperatorData *c, OperatorData *f, const int32_t *children);
+  void prolongate_add(void *fine, const void *coarse);
+  void restrict_add(void *coarse, const void *fine);
+  void interpolate(void *coarse, const void *fine);
+
+ private:
+  void run(int kind, void *fine, const void *coarse_in, void *coarse_out, const void *fine_in);
+};
+
+struct MGLevel {
+  OperatorData *op = nullptr;
+  DeviceBuffer sol, def, tmp, d, t, cg_r, cg_p, cg_h, cg_z;
+  int degree = 5;
+  double theta = 1, delta = 0, lmin = 0, lmax = 0;
+  void allocate();
+  void estimate_eigenvalues(int n_it, Reducer &red);
+  void setup_chebyshev(int deg, double range, int eig_it, Reducer &red);
+  void chebyshev(void *x, const void *b, bool zero_guess);
+};
+
+struct MGData {
+  std::vector<MGLevel> levels;
+  std::vector<TransferData *> transfers;
+  mfg_mg_params prm{};
+  Reducer red;
+  bool ready = false;
+  void setup();
+  void vmult(void *dst, const void *src, int vec_prec);
+  void v_cycle(int l);
+  void coarse_solve();
+};
+
+struct CGWorkspace {
+  DeviceBuffer r, z, p, h;
+  void ensure(size_t bytes);
+};
+
+void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, const void *b, void *x, double tol_rel,
+              int max_iter, mfg_solve_info *info, CGWorkspace &ws, Reducer &red);
+
+}  // namespace mfg
+
+struct mfg_transfer_t : mfg::TransferData {};
+struct mfg_mg_t : mfg::MGData {};

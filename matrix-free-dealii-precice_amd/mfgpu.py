@@ -73,6 +73,21 @@ SIGNATURES = {
     "mfg_operator_n_colours": (ctypes.c_int32, [_vp]),
     "mfg_operator_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mfg_operator_timing": (ctypes.c_int, [_vp, _c_dp, ctypes.POINTER(ctypes.c_int64)]),
+    "mfg_box_mesh_cells": (ctypes.c_int, [_vp, _c_i32p]),
+    "mfg_box_mesh_children": (ctypes.c_int, [_vp, _vp, _c_i32p]),
+    "mfg_transfer_create": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.POINTER(_vp)]),
+    "mfg_transfer_destroy": (ctypes.c_int, [_vp]),
+    "mfg_transfer_prolongate_add": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_transfer_restrict_add": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_transfer_interpolate": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mfg_mg_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_void_p,
+                                      ctypes.POINTER(_vp)]),
+    "mfg_mg_destroy": (ctypes.c_int, [_vp]),
+    "mfg_mg_setup": (ctypes.c_int, [_vp]),
+    "mfg_mg_vmult": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32]),
+    "mfg_mg_level_info": (ctypes.c_int, [_vp, ctypes.c_int32, _c_dp, _c_dp, _c_i32p]),
+    "mfg_cg_solve": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.c_double, _vp, _vp, ctypes.c_double,
+                                     ctypes.c_int32, ctypes.c_void_p]),
     "mfg_fill_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
     "mfg_box_mesh_fill_u0": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _c_dp]),
 }
@@ -291,3 +306,94 @@ class LaplaceOperator(_Operator):
                                     ctypes.byref(h)))
         super().__init__(h, precision, mesh)
         self.n_qp = mesh.n_cells * (mesh.degree + 1) ** 3
+
+
+# ------------------------------------------------------------ multigrid ---
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
+
+
+class MGParams(ctypes.Structure):
+    _fields_ = [("cheb_degree", ctypes.c_int32), ("smoothing_range", ctypes.c_double),
+                ("eig_cg_iterations", ctypes.c_int32), ("coarse_kind", ctypes.c_int32),
+                ("coarse_range", ctypes.c_double), ("coarse_tolerance", ctypes.c_double)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("initial_residual", ctypes.c_double),
+                ("final_residual", ctypes.c_double), ("condition_estimate", ctypes.c_double)]
+
+
+def box_children(coarse, fine):
+    import numpy as np
+    out = np.zeros(coarse.n_cells * 8, dtype=np.int32)
+    check(lib().mfg_box_mesh_children(coarse.h, fine.h, out.ctypes.data_as(_c_i32p)))
+    return out
+
+
+class Transfer:
+    """MGTransferMatrixFree between two nested levels (mf_elasticity.h:863-877)."""
+
+    def __init__(self, coarse_op, fine_op, children):
+        import numpy as np
+        ch = np.ascontiguousarray(children, dtype=np.int32)
+        h = _vp()
+        check(lib().mfg_transfer_create(coarse_op.h, fine_op.h, ch.ctypes.data_as(_c_i32p), ctypes.byref(h)))
+        self.h, self._keep = h, (coarse_op, fine_op)
+        self._destroy = lib().mfg_transfer_destroy
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def prolongate_add(self, fine, coarse):
+        check(lib().mfg_transfer_prolongate_add(self.h, _ptr(fine), _ptr(coarse)))
+
+    def restrict_add(self, coarse, fine):
+        check(lib().mfg_transfer_restrict_add(self.h, _ptr(coarse), _ptr(fine)))
+
+    def interpolate(self, coarse, fine):
+        check(lib().mfg_transfer_interpolate(self.h, _ptr(coarse), _ptr(fine)))
+
+
+class Multigrid:
+    """Multigrid + PreconditionMG with Chebyshev smoothing (mf_elasticity.h:1140-1279)."""
+
+    def __init__(self, ops, transfers, cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0,
+                 coarse_range=1e-3, coarse_tolerance=1e-3):
+        L = len(ops)
+        oa = (_vp * L)(*[o.h for o in ops])
+        ta = (_vp * max(L - 1, 1))(*([t.h for t in transfers] or [None]))
+        self.prm = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, coarse_range,
+                            coarse_tolerance)
+        h = _vp()
+        check(lib().mfg_mg_create(L, oa, ta, ctypes.byref(self.prm), ctypes.byref(h)))
+        self.h, self._keep = h, (list(ops), list(transfers))
+        self._destroy = lib().mfg_mg_destroy
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def setup(self):
+        check(lib().mfg_mg_setup(self.h))
+
+    def vmult(self, dst, src):
+        import torch
+        prec = FP64 if src.dtype == torch.float64 else FP32
+        check(lib().mfg_mg_vmult(self.h, _ptr(dst), _ptr(src), prec))
+
+    def level_info(self, level):
+        lmax, lmin, deg = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int32(0)
+        check(lib().mfg_mg_level_info(self.h, level, ctypes.byref(lmax), ctypes.byref(lmin), ctypes.byref(deg)))
+        return lmax.value, lmin.value, deg.value
+
+
+def cg_solve(A, b, x, preconditioner=PRECOND_NONE, mg=None, omega=1.0, tol_rel=1e-8, max_iter=10000):
+    """SolverCG::solve; returns SolveInfo.  Raises MfgError(4) on no convergence."""
+    info = SolveInfo()
+    st = lib().mfg_cg_solve(A.h, preconditioner, None if mg is None else mg.h, omega, _ptr(b), _ptr(x), tol_rel,
+                            max_iter, ctypes.byref(info))
+    check(st)
+    return info

@@ -351,9 +351,7 @@ void orc_cheb_eigenvalues(int kind, void *op, void *mesh, int n_it, double *lmax
   cg_eigen(A, v, n_it, *lmin, *lmax);
 }
 
-int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops, const orc_mg_params *prm, const double *bp,
-                    double *xp, double *final_rel) {
-  MGSolver mg;
+static void mg_build(MGSolver &mg, int kind, int n_levels, void **meshes, void **ops, const orc_mg_params *prm) {
   mg.L = n_levels - 1;
   mg.coarse_kind = prm->coarse_kind;
   mg.meshes.resize(n_levels);
@@ -368,11 +366,33 @@ int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops, const orc
     mg.sol[l].assign(n, 0.0); mg.def[l].assign(n, 0.0); mg.tmp[l].assign(n, 0.0);
   }
   for (int l = 0; l < n_levels; ++l) {
-    if (l == 0)  // coarse level: Chebyshev as a solver, accurate eigenvalues
+    if (l == 0 && prm->coarse_kind == 0)
       mg.cheb[0].setup(mg.ops[0], 0, 1e-3, (int)mg.ops[0].n());
     else
       mg.cheb[l].setup(mg.ops[l], prm->cheb_degree, prm->smoothing_range, prm->eig_cg_iterations);
   }
+}
+
+void orc_mg_vmult(int kind, int n_levels, void **meshes, void **ops, const orc_mg_params *prm, const double *src,
+                  double *dst, double *eig /* 3 per level: lmax, lmin, degree */) {
+  MGSolver mg;
+  mg_build(mg, kind, n_levels, meshes, ops, prm);
+  const size_t n = mg.ops[mg.L].n();
+  vector<double> s(src, src + n), d(n);
+  mg.precondition(d, s);
+  std::copy(d.begin(), d.end(), dst);
+  if (eig)
+    for (int l = 0; l < n_levels; ++l) {
+      eig[3 * l] = mg.cheb[l].lmax;
+      eig[3 * l + 1] = mg.cheb[l].lmin;
+      eig[3 * l + 2] = mg.cheb[l].degree;
+    }
+}
+
+int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops, const orc_mg_params *prm, const double *bp,
+                    double *xp, double *final_rel) {
+  MGSolver mg;
+  mg_build(mg, kind, n_levels, meshes, ops, prm);
   const LevelOp &A = mg.ops[mg.L];
   const size_t n = A.n();
   vector<double> b(bp, bp + n), x(xp, xp + n), r(n), d(n), p(n), h(n);

@@ -146,6 +146,11 @@ typedef struct {
 int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops,
                     const orc_mg_params *prm, const double *b, double *x,
                     double *final_rel_residual);
+/* One preconditioner application dst = V-cycle(src) after the same set-up;
+ * eig (optional): per level lmax, lmin, Chebyshev degree. */
+void orc_mg_vmult(int kind, int n_levels, void **meshes, void **ops,
+                  const orc_mg_params *prm, const double *src, double *dst,
+                  double *eig);
 /* plain / Jacobi-preconditioned CG (omega scales the inverse diagonal) */
 int orc_cg_solve(int kind, void *op, int precond, double omega, const double *b,
                  double *x, double tol_rel, int max_iter, double *final_rel);

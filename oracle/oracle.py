@@ -65,6 +65,8 @@ def lib():
             "orc_interpolate": (None, [_vp, _vp, _dp, _dp]),
             "orc_mg_cg_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                                ctypes.POINTER(_vp), ctypes.POINTER(MGParams), _dp, _dp, _dp]),
+            "orc_mg_vmult": (None, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(MGParams), _dp, _dp, _dp]),
             "orc_cg_solve": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                             ctypes.c_double, ctypes.c_int, _dp]),
             "orc_cheb_eigenvalues": (None, [ctypes.c_int, _vp, _vp, ctypes.c_int, _dp, _dp]),
@@ -285,6 +287,19 @@ def mg_cg_solve(meshes, ops, b, cheb_degree=5, smoothing_range=20.0, eig_cg_iter
     it = lib().orc_mg_cg_solve(ops[0].KIND, L, mh, oh, ctypes.byref(prm), dptr(np.ascontiguousarray(b)), dptr(x),
                                ctypes.byref(rel))
     return x, it, rel.value
+
+
+def mg_vmult(meshes, ops, src, cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0):
+    """One V-cycle after set-up; returns (dst, [(lmax, lmin, degree) per level])."""
+    L = len(meshes)
+    mh = (_vp * L)(*[m.h for m in meshes])
+    oh = (_vp * L)(*[o.h for o in ops])
+    prm = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, 0.0, 0)
+    dst = np.zeros_like(src)
+    eig = np.zeros(3 * L)
+    lib().orc_mg_vmult(ops[0].KIND, L, mh, oh, ctypes.byref(prm), dptr(np.ascontiguousarray(src)), dptr(dst),
+                       dptr(eig))
+    return dst, [tuple(eig[3 * l:3 * l + 3]) for l in range(L)]
 
 
 def cg_solve(op, b, precond=0, omega=1.0, tol_rel=1e-8, max_iter=10000):
