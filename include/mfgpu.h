@@ -225,6 +225,60 @@ typedef struct mfg_solve_info {
 int mfg_cg_solve(mfg_operator A, int32_t preconditioner, mfg_mg mg, double omega, const void *b, void *x,
                  double tol_rel, int32_t max_iter, mfg_solve_info *info);
 
+/* ------------------------------------------------ Newton-Newmark driver --- */
+/* Residual of the solid at displacement u and acceleration acc, on the
+ * operator's level layout (Solid::assemble_residual fast path,
+ * mf_elasticity.h:1764-1846, then constraints.set_zero, :1905): R (FP64)
+ * = -int tau F^-T : Grad w + int rho (body - acc) . w; returns ||R||_2.
+ * The operator must be FP64. */
+int mfg_solid_residual(mfg_operator op, const double *u, const double *acc, const double body[3], double *R,
+                       double *norm);
+
+/* Newmark beta/gamma time stepping with full Newton-Raphson and MG-CG linear
+ * solves (Solid::solve_nonlinear_timestep, update_acceleration/velocity,
+ * check_convergence, solve_linear_system; mf_elasticity.h:1357-1491,
+ * 1600-1620, 1981-2070).  Convergence is tested from Newton iteration
+ * min_newton on (the reference uses 3): ||R|| <= tol_f_abs or ||du|| <=
+ * tol_u_abs, or ||R||/||R_0|| <= tol_f and ||du||/||du_0|| <= tol_u. */
+typedef struct mfg_newmark_params {
+  double beta, gamma, dt;
+  double body_force[3];
+  int32_t max_newton;
+  int32_t min_newton;
+  double tol_f, tol_u, tol_f_abs, tol_u_abs;
+  double tol_lin;   /* linear solve: ||r|| <= tol_lin ||R|| */
+  int32_t max_lin;  /* CG iteration cap per linear solve */
+} mfg_newmark_params;
+typedef struct mfg_step_info {
+  int32_t newton_iterations;
+  int32_t linear_iterations;  /* summed over the Newton iterations */
+  int32_t converged;
+  double residual_0, residual_final, update_final;
+  double seconds;
+} mfg_step_info;
+typedef struct mfg_solid_problem_t *mfg_solid_problem;
+/* Level hierarchy coarse..fine (levels[n_levels-1] is the active mesh);
+ * children[l-1] connects level l-1 and l (see mfg_transfer_create).
+ * level_precision: MFG_FP64 (plain FP64 V-cycle) or MFG_FP32 (FP32 V-cycle
+ * and level caches; the coarse level, outer CG, residual and Newton update
+ * stay FP64).  material->alpha_1 is ignored (1/(beta dt^2) is used). */
+int mfg_solid_problem_create(int32_t n_levels, const mfg_level_desc *levels, const int32_t *const *children,
+                             const mfg_solid_params *material, const mfg_newmark_params *newmark,
+                             const mfg_mg_params *mg, int32_t level_precision, mfg_solid_problem *out);
+int mfg_solid_problem_destroy(mfg_solid_problem p);
+int mfg_solid_problem_set_stream(mfg_solid_problem p, void *hip_stream);
+/* Device FP64 state: total displacement u (linearisation point and
+ * old_displacement), velocity and acceleration of the last converged step.
+ * Any pointer may be NULL (left unchanged / not copied). */
+int mfg_solid_problem_set_state(mfg_solid_problem p, const double *u, const double *v, const double *a);
+int mfg_solid_problem_get_state(mfg_solid_problem p, double *u, double *v, double *a);
+/* One Newmark time step (Newton to convergence); MFG_ERR_NOCONV if
+ * max_newton iterations do not converge (the state is then left at the last
+ * iterate). */
+int mfg_solid_problem_step(mfg_solid_problem p, mfg_step_info *info);
+/* Device bytes held by the problem (operators, caches, MG, state vectors). */
+size_t mfg_solid_problem_memory(mfg_solid_problem p);
+
 /* ------------------------------------------------------ synthetic inputs --- */
 /* dst[i] = 2*U(seed,i) - 1 with U = (splitmix64(seed,i) >> 11) * 2^-53, on the
  * device (precision MFG_FP64/MFG_FP32), ordered on `stream`. */

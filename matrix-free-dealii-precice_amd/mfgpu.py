@@ -397,3 +397,91 @@ def cg_solve(A, b, x, preconditioner=PRECOND_NONE, mg=None, omega=1.0, tol_rel=1
                             max_iter, ctypes.byref(info))
     check(st)
     return info
+
+
+# ------------------------------------------------------ Newton-Newmark ---
+class NewmarkParams(ctypes.Structure):
+    _fields_ = [("beta", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
+                ("body_force", ctypes.c_double * 3), ("max_newton", ctypes.c_int32), ("min_newton", ctypes.c_int32),
+                ("tol_f", ctypes.c_double), ("tol_u", ctypes.c_double), ("tol_f_abs", ctypes.c_double),
+                ("tol_u_abs", ctypes.c_double), ("tol_lin", ctypes.c_double), ("max_lin", ctypes.c_int32)]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [("newton_iterations", ctypes.c_int32), ("linear_iterations", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("residual_0", ctypes.c_double), ("residual_final", ctypes.c_double),
+                ("update_final", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+SIGNATURES.update({
+    "mfg_solid_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_dp, _vp, _c_dp]),
+    "mfg_solid_problem_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(LevelDesc),
+                                                 ctypes.POINTER(_c_i32p), ctypes.POINTER(SolidParams),
+                                                 ctypes.POINTER(NewmarkParams), ctypes.POINTER(MGParams),
+                                                 ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "mfg_solid_problem_destroy": (ctypes.c_int, [_vp]),
+    "mfg_solid_problem_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "mfg_solid_problem_set_state": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "mfg_solid_problem_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "mfg_solid_problem_step": (ctypes.c_int, [_vp, ctypes.POINTER(StepInfo)]),
+    "mfg_solid_problem_memory": (ctypes.c_size_t, [_vp]),
+})
+
+
+def solid_residual(op, u, acc, body=(0.0, 0.0, 0.0)):
+    """(R, ||R||) of Solid::assemble_residual at (u, acc) on op's layout (FP64)."""
+    import torch
+    b = (ctypes.c_double * 3)(*body)
+    R = torch.empty_like(u)
+    nrm = ctypes.c_double(0)
+    check(lib().mfg_solid_residual(op.h, _ptr(u), _ptr(acc), b, _ptr(R), ctypes.byref(nrm)))
+    return R, nrm.value
+
+
+class SolidProblem:
+    """Newmark + Newton-Raphson + MG-CG on the device (Solid::solve_nonlinear_timestep)."""
+
+    def __init__(self, meshes, mu, nu, rho, beta=0.25, gamma=0.5, dt=1e-3, body=(0.0, 0.0, 0.0), formulation=1,
+                 level_precision=FP64, max_newton=10, min_newton=3, tol_f=1e-8, tol_u=1e-6, tol_f_abs=0.0,
+                 tol_u_abs=0.0, tol_lin=1e-6, max_lin=1000, cheb_degree=5, smoothing_range=20.0,
+                 eig_cg_iterations=30, coarse_kind=0, coarse_range=1e-3, coarse_tolerance=1e-3):
+        L = len(meshes)
+        descs = (LevelDesc * L)(*[m.desc for m in meshes])
+        self._children = [box_children(meshes[l - 1], meshes[l]) for l in range(1, L)]
+        ch = (_c_i32p * max(L - 1, 1))(*([c.ctypes.data_as(_c_i32p) for c in self._children] or [None]))
+        sp = SolidParams(mu, nu, rho, 0.0, formulation, CACHING["tensor2"])
+        nm = NewmarkParams(beta, gamma, dt, (ctypes.c_double * 3)(*body), max_newton, min_newton, tol_f, tol_u,
+                           tol_f_abs, tol_u_abs, tol_lin, max_lin)
+        mp = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, coarse_range, coarse_tolerance)
+        h = _vp()
+        check(lib().mfg_solid_problem_create(L, descs, ch, ctypes.byref(sp), ctypes.byref(nm), ctypes.byref(mp),
+                                             level_precision, ctypes.byref(h)))
+        self.h, self._meshes = h, list(meshes)
+        self._destroy = lib().mfg_solid_problem_destroy
+        self.n_dofs = meshes[-1].n_dofs
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def set_stream(self, stream):
+        check(lib().mfg_solid_problem_set_stream(self.h, ctypes.c_void_p(int(stream.cuda_stream))))
+
+    def set_state(self, u=None, v=None, a=None):
+        check(lib().mfg_solid_problem_set_state(self.h, None if u is None else _ptr(u),
+                                                None if v is None else _ptr(v), None if a is None else _ptr(a)))
+
+    def get_state(self):
+        import torch
+        u, v, a = (torch.empty(self.n_dofs, dtype=torch.float64, device="cuda") for _ in range(3))
+        check(lib().mfg_solid_problem_get_state(self.h, _ptr(u), _ptr(v), _ptr(a)))
+        return u, v, a
+
+    def step(self):
+        info = StepInfo()
+        check(lib().mfg_solid_problem_step(self.h, ctypes.byref(info)))
+        return info
+
+    def memory(self):
+        return lib().mfg_solid_problem_memory(self.h)

@@ -424,6 +424,69 @@ int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops, const orc
   return (res <= tol) ? it : -1;
 }
 
+// One Newmark time step with full Newton-Raphson and MG-CG linear solves,
+// restating Solid::solve_nonlinear_timestep (mf_elasticity.h:1400-1491),
+// check_convergence (:1357-1395), update_acceleration/velocity (:1600-1620)
+// and update_matrix_free (:1289-1353: every level re-linearised at the
+// interpolated displacement, then the GMG set-up).
+// nm = {beta, gamma, dt, bx, by, bz, tol_f, tol_u, tol_f_abs, tol_u_abs, tol_lin}
+int orc_newmark_step(int n_levels, void **meshes, void **ops, const orc_mg_params *prm, const double *nm,
+                     int max_newton, int min_newton, double *u, double *v_old, double *a_old, int *lin_its) {
+  const double beta = nm[0], gamma = nm[1], dt = nm[2];
+  const double a1 = 1.0 / (beta * dt * dt), a2 = 1.0 / (beta * dt), a3 = (1 - 2 * beta) / (2 * beta);
+  const double a4 = gamma / (beta * dt), a5 = 1 - gamma / beta, a6 = (1 - gamma / (2 * beta)) * dt;
+  const double body[3] = {nm[3], nm[4], nm[5]};
+  const double tol_f = nm[6], tol_u = nm[7], tol_f_abs = nm[8], tol_u_abs = nm[9], tol_lin = nm[10];
+  Mesh &fm = *(Mesh *)meshes[n_levels - 1];
+  const size_t n = (size_t)fm.n_dofs;
+  vector<double> du(n, 0.0), delta(n, 0.0), acc(n), R(n);
+  vector<vector<double>> ul(n_levels);
+  double res0 = 1, upd = 1, upd0 = 1;
+  int it = 0, total_lin = 0;
+  bool converged = false;
+  for (; it < max_newton; ++it) {
+    for (size_t i = 0; i < n; ++i) acc[i] = a1 * delta[i] - a2 * v_old[i] - a3 * a_old[i];
+    orc_solid_residual(ops[n_levels - 1], u, acc.data(), body, R.data());
+    const double res = std::sqrt(dot(R, R));
+    if (it == 0) res0 = res;
+    if (it >= min_newton) {
+      const double rn = res0 != 0 ? res / res0 : res, un = upd0 != 0 ? upd / upd0 : upd;
+      if (res <= tol_f_abs || upd <= tol_u_abs || (rn <= tol_f && un <= tol_u)) {
+        converged = true;
+        break;
+      }
+    }
+    ul[n_levels - 1].assign(u, u + n);
+    for (int l = n_levels - 1; l >= 1; --l) {
+      ul[l - 1].assign((size_t)((Mesh *)meshes[l - 1])->n_dofs, 0.0);
+      mg_interpolate(*(Mesh *)meshes[l - 1], *(Mesh *)meshes[l], ul[l].data(), ul[l - 1].data());
+    }
+    for (int l = 0; l < n_levels; ++l)
+      if (orc_solid_cache(ops[l], ul[l].data()) != 0) return -2;
+    orc_mg_params lp = *prm;
+    lp.tol_rel = tol_lin;
+    std::fill(du.begin(), du.end(), 0.0);
+    double rel;
+    const int li = orc_mg_cg_solve(0, n_levels, meshes, ops, &lp, R.data(), du.data(), &rel);
+    if (li < 0) return -3;
+    total_lin += li;
+    upd = std::sqrt(dot(du, du));
+    if (it == 0) upd0 = upd;
+    for (size_t i = 0; i < n; ++i) {
+      delta[i] += du[i];
+      u[i] += du[i];
+    }
+  }
+  if (lin_its) *lin_its = total_lin;
+  if (!converged) return -1;
+  for (size_t i = 0; i < n; ++i) {
+    const double vn = a4 * delta[i] + a5 * v_old[i] + a6 * a_old[i];
+    v_old[i] = vn;
+    a_old[i] = acc[i];
+  }
+  return it;
+}
+
 int orc_cg_solve(int kind, void *op, int precond, double omega, const double *bp, double *xp, double tol_rel,
                  int max_iter, double *final_rel) {
   LevelOp A;

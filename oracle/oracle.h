@@ -151,6 +151,16 @@ int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops,
 void orc_mg_vmult(int kind, int n_levels, void **meshes, void **ops,
                   const orc_mg_params *prm, const double *src, double *dst,
                   double *eig);
+/* One Newmark step, Newton + MG-CG (solid, formulation of the level ops).
+ * nm = {beta, gamma, dt, bx, by, bz, tol_f, tol_u, tol_f_abs, tol_u_abs,
+ * tol_lin}; u, v_old, a_old updated in place.  Returns the Newton iteration
+ * count, -1 if not converged, -2 det F <= 0, -3 linear solve failure.
+ * [pinned: conservation of the Newmark relations checked in tests; residual
+ *  at the converged state below the tolerance] */
+int orc_newmark_step(int n_levels, void **meshes, void **ops,
+                     const orc_mg_params *prm, const double *nm,
+                     int max_newton, int min_newton, double *u, double *v_old,
+                     double *a_old, int *lin_its);
 /* plain / Jacobi-preconditioned CG (omega scales the inverse diagonal) */
 int orc_cg_solve(int kind, void *op, int precond, double omega, const double *b,
                  double *x, double tol_rel, int max_iter, double *final_rel);

@@ -65,6 +65,9 @@ def lib():
             "orc_interpolate": (None, [_vp, _vp, _dp, _dp]),
             "orc_mg_cg_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                                ctypes.POINTER(_vp), ctypes.POINTER(MGParams), _dp, _dp, _dp]),
+            "orc_newmark_step": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                ctypes.POINTER(MGParams), _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                                _dp, ctypes.POINTER(ctypes.c_int)]),
             "orc_mg_vmult": (None, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                     ctypes.POINTER(MGParams), _dp, _dp, _dp]),
             "orc_cg_solve": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, _dp, _dp,
@@ -300,6 +303,21 @@ def mg_vmult(meshes, ops, src, cheb_degree=5, smoothing_range=20.0, eig_cg_itera
     lib().orc_mg_vmult(ops[0].KIND, L, mh, oh, ctypes.byref(prm), dptr(np.ascontiguousarray(src)), dptr(dst),
                        dptr(eig))
     return dst, [tuple(eig[3 * l:3 * l + 3]) for l in range(L)]
+
+
+def newmark_step(meshes, ops, u, v, a, beta=BETA, gamma=0.5, dt=DT, body=(0.0, 0.0, 0.0), tol_f=1e-8, tol_u=1e-6,
+                 tol_f_abs=0.0, tol_u_abs=0.0, tol_lin=1e-6, max_newton=10, min_newton=3, cheb_degree=5,
+                 smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0):
+    """One Newton-Newmark step in place on (u, v, a); returns (newton_its, linear_its)."""
+    L = len(meshes)
+    mh = (_vp * L)(*[m.h for m in meshes])
+    oh = (_vp * L)(*[o.h for o in ops])
+    prm = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, 1e-6, 1000)
+    nm = np.array([beta, gamma, dt, *body, tol_f, tol_u, tol_f_abs, tol_u_abs, tol_lin], dtype=np.float64)
+    li = ctypes.c_int(0)
+    it = lib().orc_newmark_step(L, mh, oh, ctypes.byref(prm), dptr(nm), max_newton, min_newton, dptr(u), dptr(v),
+                                dptr(a), ctypes.byref(li))
+    return it, li.value
 
 
 def cg_solve(op, b, precond=0, omega=1.0, tol_rel=1e-8, max_iter=10000):
