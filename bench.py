@@ -58,12 +58,64 @@ def cpu_baseline(seconds_budget=20.0):
                       f"as the 128x32x32 beam), median of {len(times)} vmults, {threads} OpenMP threads"}
 
 
+def mgcg_bench(mfgpu, torch, cells, p, level_precision, x_true_seed=2):
+    """MG-CG on the beam hierarchy (coarse 4x1x1 refined to `cells`): rhs =
+    A x_true with x_true from seed 2 (constrained entries zeroed), CG to
+    ||r||/||b|| <= 1e-8, Chebyshev degree 5 (range 20, 30 eigenvalue CG steps),
+    coarse level FP64 CG to 1e-3.  level_precision FP64: plain FP64;
+    FP32: FP32 V-cycle and level caches, FP64 outer CG and coarse solve."""
+    n_levels = 1
+    while (cells[0] >> n_levels) >= 4 and (cells[1] >> n_levels) >= 1 and (cells[2] >> n_levels) >= 1 and \
+            (cells[0] >> n_levels) << n_levels == cells[0]:
+        n_levels += 1
+    coarse = [c >> (n_levels - 1) for c in cells]
+    meshes = [mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], [c << l for c in coarse], p, 3, 1, 7) for l in range(n_levels)]
+    dev = torch.device("cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    precs = [mfgpu.FP64] + [level_precision] * (n_levels - 1)
+    ops = [mfgpu.NeoHookOperator(m, MU, NU, RHO, ALPHA1, precision=pr) for m, pr in zip(meshes, precs)]
+    fine64 = ops[-1] if level_precision == mfgpu.FP64 else mfgpu.NeoHookOperator(meshes[-1], MU, NU, RHO, ALPHA1)
+    trs = [mfgpu.Transfer(ops[l - 1], ops[l], mfgpu.box_children(meshes[l - 1], meshes[l])) for l in range(1, n_levels)]
+    u = torch.from_numpy(meshes[-1].u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
+    fine64.cache(u)
+    ul = [None] * n_levels
+    ul[-1] = u.to(ops[-1].dtype)
+    for l in range(n_levels - 1, 0, -1):
+        ul[l - 1] = torch.empty(meshes[l - 1].n_dofs, dtype=ops[l - 1].dtype, device=dev)
+        trs[l - 1].interpolate(ul[l - 1], ul[l])
+    for op, uu in zip(ops, ul):
+        if op is not fine64:
+            op.cache(uu)
+    mg = mfgpu.Multigrid(ops, trs, cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=1,
+                         coarse_tolerance=1e-3)
+    mg.setup()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    n = meshes[-1].n_dofs
+    xt = mfgpu.fill_uniform(x_true_seed, torch.empty(n, dtype=torch.float64, device=dev))
+    xt[torch.from_numpy(meshes[-1].constrained()).long().to(dev)] = 0.0
+    b = torch.empty_like(xt)
+    fine64.vmult(b, xt)
+    x = torch.zeros_like(xt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = mfgpu.cg_solve(fine64, b, x, mfgpu.PRECOND_MG, mg, tol_rel=1e-8, max_iter=500)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    err = float(torch.linalg.norm(x - xt) / torch.linalg.norm(xt))
+    return {"levels": n_levels, "iterations": info.iterations, "solve_s": round(t_solve, 4),
+            "setup_s": round(t_setup, 3), "rel_residual": info.final_residual / float(torch.linalg.norm(b)),
+            "err_vs_x_true": err}, x
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mg", action="store_true", help="skip the MG-CG legs")
     ap.add_argument("--degree", type=int, default=BEAM["degree"])
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
     args = ap.parse_args()
@@ -139,6 +191,17 @@ def main():
                      "canonical_bytes_per_launch": B},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
     }
+    del op, src, dst
+    torch.cuda.empty_cache()
+    if not args.no_mg:
+        r64, x64 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP64)
+        x64 = x64.cpu()
+        torch.cuda.empty_cache()
+        r32, x32 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP32)
+        r32["err_vs_fp64_solution"] = float(torch.linalg.norm(x32.cpu() - x64) / torch.linalg.norm(x64))
+        out["mgcg"] = {"workload": "MG-CG to ||r||/||b|| <= 1e-8, rhs = A x_true (seed 2), Chebyshev degree 5, "
+                                   "coarse FP64 CG to 1e-3", "fp64": r64, "mixed_fp32_vcycle": r32,
+                       "speedup_mixed_vs_fp64": round(r64["solve_s"] / r32["solve_s"], 3)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -43,6 +43,15 @@ def build(jobs=8, verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(OUT + ".tmp", OUT)
+    # C++ drop-in test program against the host mirror (include/mfgpu.hpp)
+    src = os.path.join(HERE, "..", "tests", "cpp", "drop_in.cpp")
+    if os.path.exists(src):
+        exe = os.path.join(BUILD, "drop_in_test")
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wno-unused-result", src, "-L", HERE, "-lmfgpu",
+               "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"drop-in test build failed\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {OUT}")
     return OUT
