@@ -279,6 +279,57 @@ int mfg_solid_problem_step(mfg_solid_problem p, mfg_step_info *info);
 /* Device bytes held by the problem (operators, caches, MG, state vectors). */
 size_t mfg_solid_problem_memory(mfg_solid_problem p);
 
+/* ---------------------------------------------------------- serving loop --- */
+/* Continuous-batching service of Newmark time steps for many live solid
+ * sessions (no counterpart in the reference, which steps one solid per MPI
+ * job; each step is Solid::solve_nonlinear_timestep).  Session state (u, v,
+ * a, caches, MG hierarchy) stays on the device while resident.  Requests are
+ * released at their arrival times; up to max_concurrent steps run at once,
+ * each on its own HIP stream and host thread, so their work overlaps on the
+ * device.  A request's session is admitted when its estimated device bytes
+ * fit hbm_budget_bytes; idle sessions are evicted least-recently-used (state
+ * saved to host) to make room.  Requests of one session run in arrival
+ * order.  Results are deterministic: a session's steps give identical
+ * numbers alone and in the mix. */
+typedef struct mfg_session_spec {
+  double lo[3], hi[3];
+  int32_t n_cells[3];         /* finest level; coarser levels halve while all counts stay even */
+  int32_t degree;
+  mfg_solid_params material;  /* alpha_1 ignored */
+  mfg_newmark_params newmark;
+  mfg_mg_params mg;
+  int32_t level_precision;    /* MFG_FP64 or MFG_FP32 (mixed) */
+  double u0_amplitude, u0_noise; /* initial u: amp sin sin sin + noise U[-1,1] */
+  uint64_t seed;
+} mfg_session_spec;
+typedef struct mfg_request {
+  int32_t session;
+  double arrival_s;  /* seconds after the start of the trace */
+} mfg_request;
+typedef struct mfg_request_result {
+  double arrival_s, start_s, end_s;
+  int32_t newton_iterations, linear_iterations, status;
+  double residual_final;
+} mfg_request_result;
+typedef struct mfg_server_params {
+  int32_t max_concurrent;
+  size_t hbm_budget_bytes;
+  int32_t real_time;  /* 1: honour arrival times; 0: all requests available at t = 0 */
+} mfg_server_params;
+typedef struct mfg_serve_stats {
+  double makespan_s, steps_per_s, latency_p50_s, latency_p99_s, latency_mean_s;
+  int32_t completed, failed, admissions, evictions, max_resident;
+  size_t peak_bytes;
+} mfg_serve_stats;
+/* Runs the request trace.  results: n_requests entries (request order).
+ * final_u: optional array of n_sessions host pointers receiving each
+ * session's final displacement (n_dofs doubles; entries may be NULL).  */
+int mfg_serve(const mfg_session_spec *sessions, int32_t n_sessions, const mfg_request *requests, int32_t n_requests,
+              const mfg_server_params *params, mfg_request_result *results, double *const *final_u,
+              mfg_serve_stats *stats);
+/* DoF count of a session's finest level (to size final_u buffers). */
+int64_t mfg_session_n_dofs(const mfg_session_spec *spec);
+
 /* ------------------------------------------------------ synthetic inputs --- */
 /* dst[i] = 2*U(seed,i) - 1 with U = (splitmix64(seed,i) >> 11) * 2^-53, on the
  * device (precision MFG_FP64/MFG_FP32), ordered on `stream`. */

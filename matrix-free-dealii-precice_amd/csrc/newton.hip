@@ -10,6 +10,7 @@
 #include "common.hpp"
 #include "operator.hpp"
 #include "residual_kernels.hpp"
+#include "problem.hpp"
 #include "solver.hpp"
 
 namespace mfg {
@@ -73,119 +74,183 @@ double solid_residual(const OperatorData &op, const double *u, const double *acc
   return n;
 }
 
-struct SolidProblem {
-  int L = 0;
-  int level_precision = MFG_FP64;
-  std::vector<std::unique_ptr<mfg_operator_t>> levels;
-  std::unique_ptr<mfg_operator_t> fine64;  // outer operator when levels are FP32
-  std::vector<std::unique_ptr<mfg_transfer_t>> transfers;
-  std::unique_ptr<mfg_mg_t> mg;
-  mfg_newmark_params nm{};
-  double a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
-  DeviceBuffer u, du, delta_u, v, a, v_old, a_old, R;
-  std::vector<DeviceBuffer> u_lev;
-  hipStream_t stream = nullptr;
-  CGWorkspace ws;
-  Reducer red;
+void SolidProblem::set_stream(hipStream_t s) {
+  stream = s;
+  for (auto &o : levels) o->stream = s;
+  if (fine64) fine64->stream = s;
+}
 
-  OperatorData &fine() { return fine64 ? *fine64 : *levels.back(); }
-  int64_t n() { return fine().n_dofs; }
-
-  void set_stream(hipStream_t s) {
-    stream = s;
-    for (auto &o : levels) o->stream = s;
-    if (fine64) fine64->stream = s;
+// update_matrix_free (mf_elasticity.h:1289-1353): caches and diagonals on
+// every level at the interpolated displacement, then the GMG set-up.
+void SolidProblem::update_matrix_free() {
+  OperatorData &F = fine();
+  F.cache(u.ptr, MFG_FP64);
+  const int top = L - 1;
+  if (fine64) {
+    vec_convert(levels[top]->precision, u_lev[top].ptr, MFG_FP64, u.ptr, n(), stream);
+    levels[top]->cache(u_lev[top].ptr, levels[top]->precision);
   }
-
-  // update_matrix_free (mf_elasticity.h:1289-1353): caches and diagonals on
-  // every level at the interpolated displacement, then the GMG set-up.
-  void update_matrix_free() {
-    OperatorData &F = fine();
-    F.cache(u.ptr, MFG_FP64);
-    const int top = L - 1;
-    if (fine64) {
-      vec_convert(levels[top]->precision, u_lev[top].ptr, MFG_FP64, u.ptr, n(), stream);
-      levels[top]->cache(u_lev[top].ptr, levels[top]->precision);
-    }
-    const void *u_top = fine64 ? u_lev[top].ptr : u.ptr;
-    for (int l = top; l >= 1; --l) {
-      const void *uf = l == top ? u_top : u_lev[l].ptr;
-      transfers[l - 1]->interpolate(u_lev[l - 1].ptr, uf);
-      levels[l - 1]->cache(u_lev[l - 1].ptr, levels[l - 1]->precision);
-    }
-    mg->setup();
+  const void *u_top = fine64 ? u_lev[top].ptr : u.ptr;
+  for (int l = top; l >= 1; --l) {
+    const void *uf = l == top ? u_top : u_lev[l].ptr;
+    transfers[l - 1]->interpolate(u_lev[l - 1].ptr, uf);
+    levels[l - 1]->cache(u_lev[l - 1].ptr, levels[l - 1]->precision);
   }
+  mg->setup();
+}
 
-  void step(mfg_step_info &info) {
-    const auto t0 = std::chrono::steady_clock::now();
-    const int64_t N = n();
-    OperatorData &F = fine();
-    std::memset(&info, 0, sizeof(info));
-    vec_fill(MFG_FP64, delta_u.ptr, N, 0.0, stream);
-    double res0 = 1.0, upd = 1.0, upd0 = 1.0, res = 0.0;
-    bool converged = false;
-    int it = 0;
-    for (; it < nm.max_newton; ++it) {
-      newmark_acc_kernel<double><<<(int)std::min<int64_t>((N + 255) / 256, 8192), 256, 0, stream>>>(
-          a.as<double>(), delta_u.as<double>(), v_old.as<double>(), a_old.as<double>(), a1, a2, a3, N);
-      MFG_HIP(hipGetLastError());
-      res = solid_residual(F, u.as<double>(), a.as<double>(), nm.body_force, R.as<double>(), red);
-      if (it == 0) {
-        res0 = res;
-        info.residual_0 = res;
+void SolidProblem::step(mfg_step_info &info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t N = n();
+  OperatorData &F = fine();
+  std::memset(&info, 0, sizeof(info));
+  const int grid = (int)std::min<int64_t>((N + 255) / 256, 8192);
+  vec_fill(MFG_FP64, delta_u.ptr, N, 0.0, stream);
+  double res0 = 1.0, upd = 1.0, upd0 = 1.0, res = 0.0;
+  bool converged = false;
+  int it = 0;
+  for (; it < nm.max_newton; ++it) {
+    newmark_acc_kernel<double><<<grid, 256, 0, stream>>>(a.as<double>(), delta_u.as<double>(), v_old.as<double>(),
+                                                         a_old.as<double>(), a1, a2, a3, N);
+    MFG_HIP(hipGetLastError());
+    res = solid_residual(F, u.as<double>(), a.as<double>(), nm.body_force, R.as<double>(), red);
+    if (it == 0) {
+      res0 = res;
+      info.residual_0 = res;
+    }
+    // check_convergence (mf_elasticity.h:1357-1395)
+    if (it >= nm.min_newton) {
+      const double rn = res0 != 0.0 ? res / res0 : res, un = upd0 != 0.0 ? upd / upd0 : upd;
+      if (res <= nm.tol_f_abs || upd <= nm.tol_u_abs || (rn <= nm.tol_f && un <= nm.tol_u)) {
+        converged = true;
+        break;
       }
-      // check_convergence (mf_elasticity.h:1357-1395)
-      if (it >= nm.min_newton) {
-        const double rn = res0 != 0.0 ? res / res0 : res, un = upd0 != 0.0 ? upd / upd0 : upd;
-        if (res <= nm.tol_f_abs || upd <= nm.tol_u_abs || (rn <= nm.tol_f && un <= nm.tol_u)) {
-          converged = true;
-          break;
-        }
-      }
-      update_matrix_free();
-      vec_fill(MFG_FP64, du.ptr, N, 0.0, stream);
-      mfg_solve_info si{};
-      cg_solve(F, MFG_PRECOND_MG, mg.get(), 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
-      info.linear_iterations += si.iterations;
-      upd = std::sqrt(red.dot(MFG_FP64, du.ptr, du.ptr, N, stream));
-      if (it == 0) upd0 = upd;
-      add2_kernel<double><<<(int)std::min<int64_t>((N + 255) / 256, 8192), 256, 0, stream>>>(
-          delta_u.as<double>(), u.as<double>(), du.as<double>(), N);
-      MFG_HIP(hipGetLastError());
     }
-    info.newton_iterations = it;
-    info.residual_final = res;
-    info.update_final = upd;
-    info.converged = converged;
-    if (converged) {
-      // update_velocity (:1600-1609): v = a4 du + a5 v_old + a6 a_old; then
-      // v_old <- v, a_old <- a
-      newmark_vel_kernel<double><<<(int)std::min<int64_t>((N + 255) / 256, 8192), 256, 0, stream>>>(
-          v.as<double>(), delta_u.as<double>(), v_old.as<double>(), a_old.as<double>(), a4, a5, a6, N);
-      MFG_HIP(hipGetLastError());
-      vec_convert(MFG_FP64, v_old.ptr, MFG_FP64, v.ptr, N, stream);
-      vec_convert(MFG_FP64, a_old.ptr, MFG_FP64, a.ptr, N, stream);
-    }
-    MFG_HIP(hipStreamSynchronize(stream));
-    info.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (!converged) throw Error(MFG_ERR_NOCONV, "No convergence in nonlinear solver!");
+    update_matrix_free();
+    vec_fill(MFG_FP64, du.ptr, N, 0.0, stream);
+    mfg_solve_info si{};
+    cg_solve(F, MFG_PRECOND_MG, mg.get(), 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
+    info.linear_iterations += si.iterations;
+    upd = std::sqrt(red.dot(MFG_FP64, du.ptr, du.ptr, N, stream));
+    if (it == 0) upd0 = upd;
+    add2_kernel<double><<<grid, 256, 0, stream>>>(delta_u.as<double>(), u.as<double>(), du.as<double>(), N);
+    MFG_HIP(hipGetLastError());
   }
+  info.newton_iterations = it;
+  info.residual_final = res;
+  info.update_final = upd;
+  info.converged = converged;
+  if (converged) {
+    // update_velocity (:1600-1609): v = a4 du + a5 v_old + a6 a_old; then
+    // v_old <- v, a_old <- a
+    newmark_vel_kernel<double><<<grid, 256, 0, stream>>>(v.as<double>(), delta_u.as<double>(), v_old.as<double>(),
+                                                         a_old.as<double>(), a4, a5, a6, N);
+    MFG_HIP(hipGetLastError());
+    vec_convert(MFG_FP64, v_old.ptr, MFG_FP64, v.ptr, N, stream);
+    vec_convert(MFG_FP64, a_old.ptr, MFG_FP64, a.ptr, N, stream);
+  }
+  MFG_HIP(hipStreamSynchronize(stream));
+  info.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!converged) throw Error(MFG_ERR_NOCONV, "No convergence in nonlinear solver!");
+}
 
-  size_t memory() const {
-    size_t b = u.bytes + du.bytes + delta_u.bytes + v.bytes + a.bytes + v_old.bytes + a_old.bytes + R.bytes;
-    for (auto &o : levels) b += o->memory_consumption() + o->diag.bytes + o->inv_diag.bytes + 9 * o->n_dofs * o->elem_size();
-    if (fine64) b += fine64->memory_consumption();
-    for (auto &t : transfers) b += t->lattice.bytes;
-    for (auto &x : u_lev) b += x.bytes;
-    return b;
+size_t SolidProblem::memory() const {
+  size_t b = u.bytes + du.bytes + delta_u.bytes + v.bytes + a.bytes + v_old.bytes + a_old.bytes + R.bytes;
+  for (auto &o : levels) b += o->memory_consumption() + o->diag.bytes + o->inv_diag.bytes + o->vertices.bytes;
+  for (auto &L : mg->levels)
+    for (const DeviceBuffer *x : {&L.sol, &L.def, &L.tmp, &L.d, &L.t, &L.cg_r, &L.cg_p, &L.cg_h, &L.cg_z}) b += x->bytes;
+  if (fine64) b += fine64->memory_consumption() + fine64->vertices.bytes;
+  for (auto &t : transfers) b += t->lattice.bytes;
+  for (auto &x : u_lev) b += x.bytes;
+  b += ws.r.bytes + ws.z.bytes + ws.p.bytes + ws.h.bytes;
+  return b;
+}
+
+size_t estimate_solid_problem_bytes(int n_levels, const mfg_level_desc *levels, int level_precision) {
+  size_t b = 0;
+  for (int l = 0; l < n_levels; ++l) {
+    const mfg_level_desc &d = levels[l];
+    const int64_t n3 = (int64_t)(d.degree + 1) * (d.degree + 1) * (d.degree + 1);
+    const int64_t nqp = (int64_t)d.n_cells * n3, ndof = d.n_nodes * d.n_components;
+    const size_t es = (l == 0 || level_precision == MFG_FP64) ? 8 : 4;
+    b += nqp * (18 * es + 4) + (size_t)d.n_cells * 24 * 8 + 11 * ndof * es;  // caches, map, vertices, diag + MG vectors
+    if (l > 0) b += (size_t)levels[l - 1].n_cells * (2 * d.degree + 1) * (2 * d.degree + 1) * (2 * d.degree + 1) * 4;
+    if (l == n_levels - 1) {
+      b += 12 * ndof * 8;  // state, residual, CG workspace
+      if (level_precision == MFG_FP32 && n_levels > 1) b += nqp * (18 * 8 + 4) + (size_t)d.n_cells * 24 * 8;
+    }
   }
-};
+  return b;
+}
+
+std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_desc *levels,
+                                                 const int32_t *const *children, const mfg_solid_params &material,
+                                                 const mfg_newmark_params &newmark, const mfg_mg_params &mgp,
+                                                 int level_precision, hipStream_t stream) {
+  MFG_REQUIRE(n_levels >= 1 && levels, MFG_ERR_ARGUMENT, "null argument");
+  MFG_REQUIRE(n_levels == 1 || children, MFG_ERR_ARGUMENT, "children tables required");
+  MFG_REQUIRE(level_precision == MFG_FP64 || level_precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
+  MFG_REQUIRE(newmark.dt > 0 && newmark.beta > 0, MFG_ERR_ARGUMENT, "bad Newmark parameters");
+  auto p = std::make_unique<SolidProblem>();
+  p->L = n_levels;
+  p->level_precision = level_precision;
+  p->nm = newmark;
+  const double beta = newmark.beta, gamma = newmark.gamma, dt = newmark.dt;
+  // mf_elasticity.h:240-245
+  p->a1 = 1.0 / (beta * dt * dt);
+  p->a2 = 1.0 / (beta * dt);
+  p->a3 = (1 - 2 * beta) / (2 * beta);
+  p->a4 = gamma / (beta * dt);
+  p->a5 = 1 - gamma / beta;
+  p->a6 = (1 - gamma / (2 * beta)) * dt;
+  mfg_solid_params sp = material;
+  sp.alpha_1 = p->a1;
+  sp.caching = MFG_CACHING_TENSOR2;
+  auto make_op = [&](const mfg_level_desc &d, int prec) {
+    auto op = std::make_unique<mfg_operator_t>();
+    op->kind = 0;
+    op->precision = prec;
+    op->sp = sp;
+    op->stream = stream;
+    op->setup_layout(d);
+    op->setup_solid();
+    return op;
+  };
+  for (int l = 0; l < n_levels; ++l) p->levels.push_back(make_op(levels[l], l == 0 ? MFG_FP64 : level_precision));
+  if (level_precision == MFG_FP32 && n_levels > 1) p->fine64 = make_op(levels[n_levels - 1], MFG_FP64);
+  for (int l = 1; l < n_levels; ++l) {
+    auto t = std::make_unique<mfg_transfer_t>();
+    t->build(p->levels[l - 1].get(), p->levels[l].get(), children[l - 1]);
+    p->transfers.push_back(std::move(t));
+  }
+  p->mg = std::make_unique<mfg_mg_t>();
+  p->mg->prm = mgp;
+  for (int l = 0; l < n_levels; ++l) {
+    MGLevel Lv;
+    Lv.op = p->levels[l].get();
+    p->mg->levels.push_back(std::move(Lv));
+    if (l > 0) p->mg->transfers.push_back(p->transfers[l - 1].get());
+  }
+  p->stream = stream;
+  const int64_t N = p->fine().n_dofs;
+  for (DeviceBuffer *b : {&p->u, &p->du, &p->delta_u, &p->v, &p->a, &p->v_old, &p->a_old, &p->R}) {
+    b->allocate(N * 8);
+    vec_fill(MFG_FP64, b->ptr, N, 0.0, stream);
+  }
+  p->u_lev.resize(n_levels);
+  for (int l = 0; l < n_levels; ++l) {
+    if (l == n_levels - 1 && !p->fine64) continue;  // the FP64 active level uses u itself
+    p->u_lev[l].allocate(p->levels[l]->n_dofs * p->levels[l]->elem_size());
+  }
+  MFG_HIP(hipStreamSynchronize(stream));
+  return p;
+}
 
 }  // namespace mfg
 
 using namespace mfg;
 
-struct mfg_solid_problem_t : SolidProblem {};
+
 
 extern "C" {
 
@@ -203,63 +268,10 @@ int mfg_solid_problem_create(int32_t n_levels, const mfg_level_desc *levels, con
                              const mfg_solid_params *material, const mfg_newmark_params *newmark,
                              const mfg_mg_params *mgp, int32_t level_precision, mfg_solid_problem *out) {
   return guarded([&] {
-    MFG_REQUIRE(n_levels >= 1 && levels && material && newmark && mgp && out, MFG_ERR_ARGUMENT, "null argument");
-    MFG_REQUIRE(n_levels == 1 || children, MFG_ERR_ARGUMENT, "children tables required");
-    MFG_REQUIRE(level_precision == MFG_FP64 || level_precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
-    MFG_REQUIRE(newmark->dt > 0 && newmark->beta > 0, MFG_ERR_ARGUMENT, "bad Newmark parameters");
-    auto p = std::make_unique<mfg_solid_problem_t>();
-    p->L = n_levels;
-    p->level_precision = level_precision;
-    p->nm = *newmark;
-    const double beta = newmark->beta, gamma = newmark->gamma, dt = newmark->dt;
-    // mf_elasticity.h:240-245
-    p->a1 = 1.0 / (beta * dt * dt);
-    p->a2 = 1.0 / (beta * dt);
-    p->a3 = (1 - 2 * beta) / (2 * beta);
-    p->a4 = gamma / (beta * dt);
-    p->a5 = 1 - gamma / beta;
-    p->a6 = (1 - gamma / (2 * beta)) * dt;
-    mfg_solid_params sp = *material;
-    sp.alpha_1 = p->a1;
-    sp.caching = MFG_CACHING_TENSOR2;
-    for (int l = 0; l < n_levels; ++l) {
-      const int prec = (l == 0) ? MFG_FP64 : level_precision;
-      mfg_operator op = nullptr;
-      const int st = mfg_solid_create(&levels[l], &sp, prec, &op);
-      if (st) throw Error(st, mfg_last_error());
-      p->levels.emplace_back(op);
-    }
-    if (level_precision == MFG_FP32 && n_levels > 1) {
-      mfg_operator op = nullptr;
-      const int st = mfg_solid_create(&levels[n_levels - 1], &sp, MFG_FP64, &op);
-      if (st) throw Error(st, mfg_last_error());
-      p->fine64.reset(op);
-    }
-    for (int l = 1; l < n_levels; ++l) {
-      auto t = std::make_unique<mfg_transfer_t>();
-      t->build(p->levels[l - 1].get(), p->levels[l].get(), children[l - 1]);
-      p->transfers.push_back(std::move(t));
-    }
-    p->mg = std::make_unique<mfg_mg_t>();
-    p->mg->prm = *mgp;
-    for (int l = 0; l < n_levels; ++l) {
-      MGLevel Lv;
-      Lv.op = p->levels[l].get();
-      p->mg->levels.push_back(std::move(Lv));
-      if (l > 0) p->mg->transfers.push_back(p->transfers[l - 1].get());
-    }
-    const int64_t N = p->fine().n_dofs;
-    for (DeviceBuffer *b : {&p->u, &p->du, &p->delta_u, &p->v, &p->a, &p->v_old, &p->a_old, &p->R}) {
-      b->allocate(N * 8);
-      vec_fill(MFG_FP64, b->ptr, N, 0.0, nullptr);
-    }
-    p->u_lev.resize(n_levels);
-    for (int l = 0; l < n_levels; ++l) {
-      if (l == n_levels - 1 && !p->fine64) continue;  // the FP64 active level uses u itself
-      p->u_lev[l].allocate(p->levels[l]->n_dofs * p->levels[l]->elem_size());
-    }
-    MFG_HIP(hipDeviceSynchronize());
-    *out = p.release();
+    MFG_REQUIRE(levels && material && newmark && mgp && out, MFG_ERR_ARGUMENT, "null argument");
+    auto h = std::make_unique<mfg_solid_problem_t>();
+    h->p = make_solid_problem(n_levels, levels, children, *material, *newmark, *mgp, level_precision, nullptr);
+    *out = h.release();
   });
 }
 
@@ -270,29 +282,31 @@ int mfg_solid_problem_destroy(mfg_solid_problem p) {
 int mfg_solid_problem_set_stream(mfg_solid_problem p, void *s) {
   return guarded([&] {
     MFG_REQUIRE(p, MFG_ERR_ARGUMENT, "null problem");
-    p->set_stream((hipStream_t)s);
+    p->p->set_stream((hipStream_t)s);
   });
 }
 
 int mfg_solid_problem_set_state(mfg_solid_problem p, const double *u, const double *v, const double *a) {
   return guarded([&] {
     MFG_REQUIRE(p, MFG_ERR_ARGUMENT, "null problem");
-    const int64_t N = p->n();
-    if (u) vec_convert(MFG_FP64, p->u.ptr, MFG_FP64, u, N, p->stream);
-    if (v) vec_convert(MFG_FP64, p->v_old.ptr, MFG_FP64, v, N, p->stream);
-    if (a) vec_convert(MFG_FP64, p->a_old.ptr, MFG_FP64, a, N, p->stream);
-    if (v) vec_convert(MFG_FP64, p->v.ptr, MFG_FP64, v, N, p->stream);
-    if (a) vec_convert(MFG_FP64, p->a.ptr, MFG_FP64, a, N, p->stream);
+    SolidProblem &q = *p->p;
+    const int64_t N = q.n();
+    if (u) vec_convert(MFG_FP64, q.u.ptr, MFG_FP64, u, N, q.stream);
+    if (v) vec_convert(MFG_FP64, q.v_old.ptr, MFG_FP64, v, N, q.stream);
+    if (a) vec_convert(MFG_FP64, q.a_old.ptr, MFG_FP64, a, N, q.stream);
+    if (v) vec_convert(MFG_FP64, q.v.ptr, MFG_FP64, v, N, q.stream);
+    if (a) vec_convert(MFG_FP64, q.a.ptr, MFG_FP64, a, N, q.stream);
   });
 }
 
 int mfg_solid_problem_get_state(mfg_solid_problem p, double *u, double *v, double *a) {
   return guarded([&] {
     MFG_REQUIRE(p, MFG_ERR_ARGUMENT, "null problem");
-    const int64_t N = p->n();
-    if (u) vec_convert(MFG_FP64, u, MFG_FP64, p->u.ptr, N, p->stream);
-    if (v) vec_convert(MFG_FP64, v, MFG_FP64, p->v_old.ptr, N, p->stream);
-    if (a) vec_convert(MFG_FP64, a, MFG_FP64, p->a_old.ptr, N, p->stream);
+    SolidProblem &q = *p->p;
+    const int64_t N = q.n();
+    if (u) vec_convert(MFG_FP64, u, MFG_FP64, q.u.ptr, N, q.stream);
+    if (v) vec_convert(MFG_FP64, v, MFG_FP64, q.v_old.ptr, N, q.stream);
+    if (a) vec_convert(MFG_FP64, a, MFG_FP64, q.a_old.ptr, N, q.stream);
   });
 }
 
@@ -301,10 +315,10 @@ int mfg_solid_problem_step(mfg_solid_problem p, mfg_step_info *info) {
   if (!info) info = &dummy;
   return guarded([&] {
     MFG_REQUIRE(p, MFG_ERR_ARGUMENT, "null problem");
-    p->step(*info);
+    p->p->step(*info);
   });
 }
 
-size_t mfg_solid_problem_memory(mfg_solid_problem p) { return p ? p->memory() : 0; }
+size_t mfg_solid_problem_memory(mfg_solid_problem p) { return p ? p->p->memory() : 0; }
 
 }  // extern "C"

@@ -428,6 +428,79 @@ SIGNATURES.update({
 })
 
 
+class SessionSpec(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3), ("n_cells", ctypes.c_int32 * 3),
+                ("degree", ctypes.c_int32), ("material", SolidParams), ("newmark", NewmarkParams),
+                ("mg", MGParams), ("level_precision", ctypes.c_int32), ("u0_amplitude", ctypes.c_double),
+                ("u0_noise", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("session", ctypes.c_int32), ("arrival_s", ctypes.c_double)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double),
+                ("newton_iterations", ctypes.c_int32), ("linear_iterations", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("residual_final", ctypes.c_double)]
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("max_concurrent", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_size_t),
+                ("real_time", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("makespan_s", ctypes.c_double), ("steps_per_s", ctypes.c_double),
+                ("latency_p50_s", ctypes.c_double), ("latency_p99_s", ctypes.c_double),
+                ("latency_mean_s", ctypes.c_double), ("completed", ctypes.c_int32), ("failed", ctypes.c_int32),
+                ("admissions", ctypes.c_int32), ("evictions", ctypes.c_int32), ("max_resident", ctypes.c_int32),
+                ("peak_bytes", ctypes.c_size_t)]
+
+
+SIGNATURES.update({
+    "mfg_serve": (ctypes.c_int, [ctypes.POINTER(SessionSpec), ctypes.c_int32, ctypes.POINTER(Request),
+                                 ctypes.c_int32, ctypes.POINTER(ServerParams), ctypes.POINTER(RequestResult),
+                                 ctypes.POINTER(_c_dp), ctypes.POINTER(ServeStats)]),
+    "mfg_session_n_dofs": (ctypes.c_int64, [ctypes.POINTER(SessionSpec)]),
+})
+
+
+def session_spec(lo, hi, n_cells, degree, mu, nu, rho, beta=0.25, gamma=0.5, dt=1e-3, body=(0.0, 0.0, 0.0),
+                 level_precision=FP32, tol_f=1e-8, tol_u=1e-6, tol_lin=1e-6, max_newton=10, min_newton=3,
+                 cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=1, u0_amplitude=1e-3,
+                 u0_noise=1e-5, seed=0):
+    s = SessionSpec()
+    s.lo[:] = list(lo)
+    s.hi[:] = list(hi)
+    s.n_cells[:] = list(n_cells)
+    s.degree = degree
+    s.material = SolidParams(mu, nu, rho, 0.0, 1, CACHING["tensor2"])
+    s.newmark = NewmarkParams(beta, gamma, dt, (ctypes.c_double * 3)(*body), max_newton, min_newton, tol_f, tol_u,
+                              0.0, 0.0, tol_lin, 1000)
+    s.mg = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, 1e-3, 1e-3)
+    s.level_precision = level_precision
+    s.u0_amplitude, s.u0_noise, s.seed = u0_amplitude, u0_noise, seed
+    return s
+
+
+def serve(specs, requests, max_concurrent=4, hbm_budget_bytes=64 << 30, real_time=True, final_states=False):
+    """Runs mfg_serve; returns (results, stats, final_u list or None)."""
+    import numpy as np
+    n_s, n_r = len(specs), len(requests)
+    sa = (SessionSpec * n_s)(*specs)
+    ra = (Request * n_r)(*[Request(s, t) for s, t in requests])
+    res = (RequestResult * max(n_r, 1))()
+    prm = ServerParams(max_concurrent, hbm_budget_bytes, 1 if real_time else 0)
+    st = ServeStats()
+    finals, fptr = None, None
+    if final_states:
+        finals = [np.zeros(lib().mfg_session_n_dofs(ctypes.byref(sa[i]))) for i in range(n_s)]
+        fptr = (_c_dp * n_s)(*[f.ctypes.data_as(_c_dp) for f in finals])
+    check(lib().mfg_serve(sa, n_s, ra, n_r, ctypes.byref(prm), res, fptr, ctypes.byref(st)))
+    return list(res)[:n_r], st, finals
+
+
 def solid_residual(op, u, acc, body=(0.0, 0.0, 0.0)):
     """(R, ||R||) of Solid::assemble_residual at (u, acc) on op's layout (FP64)."""
     import torch
