@@ -411,7 +411,7 @@ void MGData::setup() {
   for (size_t l = 0; l < levels.size(); ++l) {
     MGLevel &L = levels[l];
     if (l == 0) {
-      if (prm.coarse_kind == 0) L.setup_chebyshev(0, prm.coarse_range, (int)L.op->n_dofs, red);
+      if (coarse_kind() == 0) L.setup_chebyshev(0, prm.coarse_range, (int)L.op->n_dofs, red);
       else L.setup_chebyshev(prm.cheb_degree, prm.smoothing_range, prm.eig_cg_iterations, red);
     } else {
       L.setup_chebyshev(prm.cheb_degree, prm.smoothing_range, prm.eig_cg_iterations, red);
@@ -424,7 +424,7 @@ void MGData::setup() {
 // 1e-10), max(m, 100) iterations, preconditioned by the level-0 Chebyshev.
 void MGData::coarse_solve() {
   MGLevel &L = levels[0];
-  if (prm.coarse_kind == 0) {
+  if (coarse_kind() == 0) {
     L.chebyshev(L.sol.ptr, L.def.ptr, true);
     return;
   }

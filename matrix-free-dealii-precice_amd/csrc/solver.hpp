@@ -57,6 +57,10 @@ struct MGData {
   mfg_mg_params prm{};
   Reducer red;
   bool ready = false;
+  // A single-level hierarchy is its own coarse solve; an inner CG would make
+  // the preconditioner non-linear (outer CG stagnates), so the Chebyshev
+  // solver mode is used there whatever prm.coarse_kind says.
+  int coarse_kind() const { return levels.size() == 1 ? 0 : prm.coarse_kind; }
   void setup();
   void vmult(void *dst, const void *src, int vec_prec);
   void v_cycle(int l);

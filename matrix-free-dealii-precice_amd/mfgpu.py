@@ -468,7 +468,7 @@ SIGNATURES.update({
 
 def session_spec(lo, hi, n_cells, degree, mu, nu, rho, beta=0.25, gamma=0.5, dt=1e-3, body=(0.0, 0.0, 0.0),
                  level_precision=FP32, tol_f=1e-8, tol_u=1e-6, tol_lin=1e-6, max_newton=10, min_newton=3,
-                 cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=1, u0_amplitude=1e-3,
+                 cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0, u0_amplitude=1e-3,
                  u0_noise=1e-5, seed=0):
     s = SessionSpec()
     s.lo[:] = list(lo)

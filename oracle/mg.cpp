@@ -353,7 +353,9 @@ void orc_cheb_eigenvalues(int kind, void *op, void *mesh, int n_it, double *lmax
 
 static void mg_build(MGSolver &mg, int kind, int n_levels, void **meshes, void **ops, const orc_mg_params *prm) {
   mg.L = n_levels - 1;
-  mg.coarse_kind = prm->coarse_kind;
+  // a single level is its own coarse solve: Chebyshev solver mode keeps the
+  // preconditioner linear (an inner CG would not)
+  mg.coarse_kind = n_levels == 1 ? 0 : prm->coarse_kind;
   mg.meshes.resize(n_levels);
   mg.ops.resize(n_levels);
   mg.cheb.resize(n_levels);
@@ -366,7 +368,7 @@ static void mg_build(MGSolver &mg, int kind, int n_levels, void **meshes, void *
     mg.sol[l].assign(n, 0.0); mg.def[l].assign(n, 0.0); mg.tmp[l].assign(n, 0.0);
   }
   for (int l = 0; l < n_levels; ++l) {
-    if (l == 0 && prm->coarse_kind == 0)
+    if (l == 0 && mg.coarse_kind == 0)
       mg.cheb[0].setup(mg.ops[0], 0, 1e-3, (int)mg.ops[0].n());
     else
       mg.cheb[l].setup(mg.ops[l], prm->cheb_degree, prm->smoothing_range, prm->eig_cg_iterations);
