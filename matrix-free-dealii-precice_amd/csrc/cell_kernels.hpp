@@ -6,9 +6,12 @@
 // each evaluate/integrate pass go through an LDS tile per cell; the
 // collocation derivative at the Gauss points turns the three gradient
 // directions into one LDS round trip.  Gather is a coalesced read of the
-// int32 cell->node map followed by per-node vector loads; the scatter is a
-// hardware FP64/FP32 atomic add (constrained dofs are skipped via the
-// constraint bits stored above bit 28 of every map entry).
+// int32 cell->node map followed by per-node vector loads.  Cells come in
+// colours (no two cells of a colour share a node), one launch per colour, so
+// the scatter is a plain read-modify-write; the first colour that touches a
+// node stores instead (bit 31 of its map entry), which makes vmult need no
+// zero fill.  Constrained dofs are skipped via the constraint bits stored
+// above bit 28 of every map entry.
 //
 // Algorithmic reference: FEEvaluation::evaluate/integrate with
 // values|gradients as used by NeoHookOperator::do_operation_on_cell
@@ -36,8 +39,6 @@ __device__ inline int xcd_remap(int b, int G) {
   return x * q + (x < r ? x : r) + l;
 }
 
-__device__ inline void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ inline void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 
 // Per-thread view of the cell tile in LDS.
 template <int N, typename Number>
@@ -361,7 +362,7 @@ struct ApplyArgs {
 
 template <int P, int CPB, typename Number, class QOp>
 __global__ void __launch_bounds__((block_threads<P, CPB>()))
-    apply_cells(ApplyArgs<Number> a, QOp qop) {
+    apply_cells(ApplyArgs<Number> a, QOp qop, int assign) {
   constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC, NT = block_threads<P, CPB>();
   __shared__ Number s_shape[N * N], s_colloc[N * N];
   __shared__ Number s_tile[CPB * 3 * NC * N3];
@@ -411,9 +412,13 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
   Number out[NC];
   integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
   if (active) {
+    const bool first = assign && (entry >> 31);  // first colour touching the node stores
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      if (!((entry >> (kConstraintShift + c)) & 1u)) a.dst[node * NC + c] += out[c];
+      if (!((entry >> (kConstraintShift + c)) & 1u)) {
+        if (first) a.dst[node * NC + c] = out[c];
+        else a.dst[node * NC + c] += out[c];
+      }
   }
 }
 

@@ -43,25 +43,30 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
   a.constrained = op.constrained.as<int32_t>();
   a.n_constrained = op.n_constrained;
   constexpr int NT = block_threads<P, CPB>();
-  for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
-    a.cell_begin = op.colour_start[col];
-    a.cell_end = op.colour_start[col + 1];
-    a.fix_mode = col == 0 ? fix_mode : 0;
-    const int grid = blocks_for(a.cell_end - a.cell_begin, CPB);
-    if (grid == 0) continue;
-    if (op.kind == 0) {
-      if (op.sp.formulation == 1) {
-        QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
-        apply_cells<P, CPB, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      } else {
-        QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
-        apply_cells<P, CPB, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      }
-    } else {
-      QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
-      apply_cells<P, CPB, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+  const int assign = fix_mode == 1 ? 1 : 0;  // vmult: first touch stores, no zero fill
+  // one workgroup per group of CPB cells, one launch per colour
+  auto launch_simple = [&](auto kernel, auto q) {
+    for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
+      a.cell_begin = op.colour_start[col];
+      a.cell_end = op.colour_start[col + 1];
+      a.fix_mode = col == 0 ? fix_mode : 0;
+      const int groups = blocks_for(a.cell_end - a.cell_begin, CPB);
+      if (groups == 0) continue;
+      kernel<<<groups, NT, 0, op.stream>>>(a, q, assign);
+      MFG_HIP(hipGetLastError());
     }
-    MFG_HIP(hipGetLastError());
+  };
+  if (op.kind == 0) {
+    if (op.sp.formulation == 1) {
+      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+      launch_simple(apply_cells<P, CPB, Number, QSolidT2F1<Number>>, q);
+    } else {
+      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
+      launch_simple(apply_cells<P, CPB, Number, QSolidT2F0<Number>>, q);
+    }
+  } else {
+    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+    launch_simple(apply_cells<P, CPB, Number, QHeat<Number>>, q);
   }
 }
 
@@ -144,17 +149,11 @@ static void diagonal_t(OperatorData &op) {
 void OperatorData::apply(const void *src, void *dst, bool add) const {
   MFG_REQUIRE(ready, MFG_ERR_STATE, "operator used before cache()");
   MFG_REQUIRE(src && dst, MFG_ERR_ARGUMENT, "null vector");
-  if (precision == MFG_FP64) {
-    if (!add) fill<double>((double *)dst, n_dofs, 0.0, stream);
-    timing_begin();
-    apply_t<double>(*this, src, dst, add ? 2 : 1);
-    timing_end();
-  } else {
-    if (!add) fill<float>((float *)dst, n_dofs, 0.0f, stream);
-    timing_begin();
-    apply_t<float>(*this, src, dst, add ? 2 : 1);
-    timing_end();
-  }
+  // vmult needs no zero fill: the first colour touching a node stores
+  timing_begin();
+  if (precision == MFG_FP64) apply_t<double>(*this, src, dst, add ? 2 : 1);
+  else apply_t<float>(*this, src, dst, add ? 2 : 1);
+  timing_end();
 }
 
 // Event pairs are reused; at most kMaxEvents launches are kept between
@@ -336,11 +335,16 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
   // device layout in colour-major cell order
   std::vector<uint32_t> map((size_t)nqp);
   std::vector<double> vert((size_t)n_cells * 24);
+  std::vector<char> touched((size_t)n_nodes, 0);
   for (int c = 0; c < n_cells; ++c) {
     const int64_t o = perm[c];
     for (int x = 0; x < N3; ++x) {
       const int32_t node = d.cell_nodes[o * N3 + x];
-      map[(size_t)c * N3 + x] = (uint32_t)node | ((uint32_t)cmask[node] << kNodeBits);
+      // bit 31: first (cell, node) pair in launch order, i.e. the first
+      // colour that writes this node stores instead of adding
+      const uint32_t first = touched[node] ? 0u : (1u << 31);
+      touched[node] = 1;
+      map[(size_t)c * N3 + x] = (uint32_t)node | ((uint32_t)cmask[node] << kNodeBits) | first;
     }
     for (int x = 0; x < 24; ++x) vert[(size_t)c * 24 + x] = d.cell_vertices[o * 24 + x];
   }
