@@ -183,8 +183,9 @@ int mfg_transfer_interpolate(mfg_transfer t, void *coarse, const void *fine);
  * inverse diagonal, interval [1.2 lmax/smoothing_range, 1.2 lmax], lmax from
  * eig_cg_iterations Jacobi-preconditioned CG steps.  Level 0: coarse_kind 0 =
  * Chebyshev as a solver (range coarse_range, degree from the estimate, m()
- * eigenvalue iterations); 1 = CG to relative coarse_tolerance (absolute
- * 1e-10, max(m,100) steps) preconditioned by the level-0 Chebyshev. */
+ * eigenvalue iterations, or min(m(), coarse_eig_iterations) when that is
+ * > 0); 1 = CG to relative coarse_tolerance (absolute 1e-10, max(m,100)
+ * steps) preconditioned by the level-0 Chebyshev. */
 typedef struct mfg_mg_params {
   int32_t cheb_degree;
   double smoothing_range;
@@ -192,6 +193,7 @@ typedef struct mfg_mg_params {
   int32_t coarse_kind;
   double coarse_range;
   double coarse_tolerance;
+  int32_t coarse_eig_iterations; /* 0: m() (the reference's choice) */
 } mfg_mg_params;
 typedef struct mfg_mg_t *mfg_mg;
 /* ops[0..n_levels-1] coarse to fine; transfers[l-1] connects level l-1 and l.
@@ -329,6 +331,74 @@ int mfg_serve(const mfg_session_spec *sessions, int32_t n_sessions, const mfg_re
               mfg_serve_stats *stats);
 /* DoF count of a session's finest level (to size final_u buffers). */
 int64_t mfg_session_n_dofs(const mfg_session_spec *spec);
+
+/* ------------------------------------------------------ batched instances --- */
+/* Many independent operators (heat and solid, degrees 1..6, uneven meshes)
+ * applied in one launch (no counterpart in the reference, which serves one
+ * operator per MatrixFree object; each instance's arithmetic is that of
+ * NeoHookOperator::vmult, mf_nh_operator.h:543-625, or LaplaceOperator,
+ * heat_transfer.h:81-250, and batched results equal per-instance results).
+ * Cell work is cut into tasks of up to cells_per_task cells of one colour of
+ * one instance.  Schedules: MFG_SCHEDULE_QUEUE, one persistent launch per
+ * application (workgroups take tasks from an atomic ticket; a task waits for
+ * its instance's previous colour), or MFG_SCHEDULE_COLOUR, one launch per
+ * colour.  The tuner times both and the cells per task of every
+ * (kind, degree) present, and keeps the fastest. */
+#define MFG_SCHEDULE_COLOUR 0
+#define MFG_SCHEDULE_QUEUE 1
+#define MFG_BATCH_SOLID_F1 0 /* tensor2, formulation 1 */
+#define MFG_BATCH_SOLID_F0 1 /* tensor2, formulation 0 */
+#define MFG_BATCH_HEAT 2
+typedef struct mfg_batch_tuning {
+  int32_t schedule;
+  int32_t cells_per_task[3][7]; /* [kind][degree] */
+  int32_t waves_per_simd;       /* register budget of the kernel (2 or 4) */
+  double ms_colour, ms_queue;   /* one application of the whole batch */
+} mfg_batch_tuning;
+typedef struct mfg_batch_t *mfg_batch;
+/* All operators must be cached and share one precision. */
+int mfg_batch_create(int32_t n, const mfg_operator *ops, mfg_batch *out);
+int mfg_batch_destroy(mfg_batch b);
+int mfg_batch_set_stream(mfg_batch b, void *hip_stream);
+int mfg_batch_set_schedule(mfg_batch b, int32_t schedule);
+/* cells = 0 restores the default (as many as fit 512 threads). */
+int mfg_batch_set_cells_per_task(mfg_batch b, int32_t kind, int32_t degree, int32_t cells);
+/* dst[i] = A_i src[i] (vmult) / dst[i] += A_i src[i] (vmult_add) for all i. */
+int mfg_batch_vmult(mfg_batch b, void *const *dst, const void *const *src);
+int mfg_batch_vmult_add(mfg_batch b, void *const *dst, const void *const *src);
+/* Waits for the batch's stream and reports a kernel wait timeout. */
+int mfg_batch_synchronize(mfg_batch b);
+/* Autotuner (times vmult on the given vectors); result may be NULL. */
+int mfg_batch_tune(mfg_batch b, const void *const *src, void *const *dst, int32_t repetitions,
+                   mfg_batch_tuning *result);
+int mfg_batch_get_tuning(mfg_batch b, mfg_batch_tuning *result);
+
+/* MG-preconditioned CG on many instances in lockstep (the batched form of
+ * mfg_cg_solve with MFG_PRECOND_MG).  fine[i] is instance i's operator
+ * (one precision for all, normally FP64); mgs[i] its set-up hierarchy whose
+ * finest level has the same mesh (levels may be FP32) and whose coarse
+ * solver is Chebyshev (coarse_kind 0).  The V-cycles of all instances run
+ * level by level (levels aligned at the coarse end) as one fixed program of
+ * batched launches, captured once into a hipGraph and replayed (use_graph,
+ * default on).  Every instance performs the operation sequence of its own
+ * mfg_mg_vmult / mfg_cg_solve. */
+typedef struct mfg_batch_solver_t *mfg_batch_solver;
+int mfg_batch_solver_create(int32_t n, const mfg_operator *fine, const mfg_mg *mgs, mfg_batch_solver *out);
+int mfg_batch_solver_destroy(mfg_batch_solver s);
+int mfg_batch_solver_use_graph(mfg_batch_solver s, int32_t on);
+int mfg_batch_solver_set_schedule(mfg_batch_solver s, int32_t schedule);
+/* Tunes the fine-operator and level-operator launches; outputs may be NULL. */
+int mfg_batch_solver_tune(mfg_batch_solver s, int32_t repetitions, mfg_batch_tuning *fine,
+                          mfg_batch_tuning *levels);
+/* dst[i] = V-cycle_i(src[i]) (fine precision vectors). */
+int mfg_batch_solver_precondition(mfg_batch_solver s, void *const *dst, const void *const *src);
+/* Solves A_i x_i = b_i to ||r_i|| <= tol_rel ||b_i||; x holds the initial
+ * guesses.  status[i]: MFG_OK, MFG_ERR_NOCONV or MFG_ERR_NUMERIC (breakdown). */
+int mfg_batch_solver_solve(mfg_batch_solver s, const void *const *b, void *const *x, double tol_rel,
+                           int32_t max_iter, mfg_solve_info *info, int32_t *status);
+/* Launches per V-cycle program and nodes of the captured graph (0 before
+ * the first graph replay). */
+int mfg_batch_solver_info(mfg_batch_solver s, int32_t *vcycle_launches, int32_t *graph_nodes);
 
 /* ------------------------------------------------------ synthetic inputs --- */
 /* dst[i] = 2*U(seed,i) - 1 with U = (splitmix64(seed,i) >> 11) * 2^-53, on the

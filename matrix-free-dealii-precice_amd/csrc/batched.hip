@@ -1,0 +1,1214 @@
+// Batched operator application and MG-preconditioned CG over many
+// independent instances (heat and solid, degrees 1..6, uneven meshes).
+//
+// Host side of the batched kernels: task tables per set of instances, the
+// per-degree cells-per-task autotuner, the batched V-cycle as a fixed
+// program of launches (optionally captured once into a hipGraph and
+// replayed), and CG in lockstep over all instances with one host
+// synchronisation per reduction.  Every instance runs exactly the operation
+// sequence of its own MGData::vmult / cg_solve, so batched results equal the
+// per-instance results.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batched_kernels.hpp"
+#include "common.hpp"
+#include "operator.hpp"
+#include "solver.hpp"
+
+namespace mfg {
+
+static int code_of(const OperatorData &op) {
+  const int kind = op.kind == 1 ? kHeat : (op.sp.formulation == 1 ? kSolidF1 : kSolidF0);
+  return kind * 8 + op.p;
+}
+static int max_cpb(int p) { return kBatchThreads / ((p + 1) * (p + 1) * (p + 1)); }
+
+struct Tuning {
+  int cpb[24] = {};  // per code; 0 = the largest that fits kBatchThreads
+  int queue = 1;
+  int waves = 2;     // register budget of the kernel: 2 or 4 waves per SIMD (FP32 only)
+  double ms_colour = 0, ms_queue = 0;
+  int cpb_for(int code) const {
+    const int m = max_cpb(code & 7), c = cpb[code];
+    return c > 0 && c <= m ? c : m;
+  }
+};
+
+template <class T>
+static void upload_vec(DeviceBuffer &buf, const std::vector<T> &v, hipStream_t s) {
+  upload(buf, v.data(), v.size(), s);
+}
+
+static int device_cus() {
+  static const int cus = [] {
+    int dev = 0, n = 0;
+    MFG_HIP(hipGetDevice(&dev));
+    MFG_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    return n;
+  }();
+  return cus;
+}
+
+template <typename Number, int WAVES>
+static int queue_capacity() {
+  static const int cap = [] {
+    int nb = 0;
+    MFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batched_apply_kernel<Number, WAVES>, kBatchThreads, 0));
+    return std::max(1, nb) * device_cus();
+  }();
+  return cap;
+}
+
+// ------------------------------------------------------- apply groups ---
+// Task table over a set of instances of one precision.
+struct ApplyGroup {
+  int prec = MFG_FP64;
+  std::vector<const OperatorData *> ops;
+  std::vector<int> ids;
+  DeviceBuffer inst, tasks, counters;
+  std::vector<int> colour_tasks;  // tasks of colour k: [colour_tasks[k], colour_tasks[k+1])
+  int n_tasks = 0, n_slots = 0;
+
+  ApplyGroup(std::vector<const OperatorData *> o, std::vector<int> id) : ops(std::move(o)), ids(std::move(id)) {
+    MFG_REQUIRE(!ops.empty(), MFG_ERR_ARGUMENT, "empty batch");
+    prec = ops[0]->precision;
+    for (const OperatorData *op : ops) {
+      MFG_REQUIRE(op && op->ready, MFG_ERR_STATE, "batched operator used before cache()");
+      MFG_REQUIRE(op->precision == prec, MFG_ERR_ARGUMENT, "batched operators must share one precision");
+      MFG_REQUIRE(op->p >= 1 && op->p <= kMaxDegree, MFG_ERR_ARGUMENT, "unsupported degree");
+    }
+  }
+
+  template <typename Number>
+  void build_t(const Tuning &tu, hipStream_t s) {
+    const int n = (int)ops.size();
+    std::vector<BatchInst<Number>> hi(n);
+    int n_col = 0;
+    for (int i = 0; i < n; ++i) {
+      const OperatorData &op = *ops[i];
+      BatchInst<Number> &e = hi[i];
+      e.cell_nodes = op.cell_nodes.as<uint32_t>();
+      e.qd = op.qd.as<Number>();
+      e.constrained = op.constrained.as<int32_t>();
+      e.shape = op.tab_shape.as<Number>();
+      e.colloc = op.tab_colloc.as<Number>();
+      e.nqp = op.nqp;
+      e.n_constrained = op.n_constrained;
+      e.code = code_of(op);
+      e.cpb = tu.cpb_for(e.code);
+      e.id = ids[i];
+      e.prm[0] = e.prm[1] = e.prm[2] = Number(0);
+      if (op.kind == 1) {
+        e.prm[0] = Number(op.dt);
+      } else if (op.sp.formulation == 1) {
+        e.prm[0] = Number(2.0 * op.lambda);
+        e.prm[1] = Number(op.rho_alpha);
+      } else {
+        e.prm[0] = Number(op.kappa);
+        e.prm[1] = Number(op.c_1);
+        e.prm[2] = Number(op.rho_alpha);
+      }
+      n_col = std::max(n_col, (int)op.colour_start.size() - 1);
+    }
+    std::vector<BatchTask> tv;
+    std::vector<int> last_slot(n, -1), last_count(n, 0);
+    colour_tasks.assign(1, 0);
+    n_slots = 0;
+    for (int col = 0; col < n_col; ++col) {
+      for (int i = 0; i < n; ++i) {
+        const OperatorData &op = *ops[i];
+        const int ncol = (int)op.colour_start.size() - 1;
+        const int cb = col < ncol ? op.colour_start[col] : 0, ce = col < ncol ? op.colour_start[col + 1] : 0;
+        const int cpb = hi[i].cpb;
+        int groups = (ce - cb + cpb - 1) / cpb;
+        const bool fix_only = col == 0 && groups == 0 && op.n_constrained > 0;
+        if (fix_only) groups = 1;
+        if (groups == 0) continue;
+        const int slot = n_slots++;
+        for (int g = 0; g < groups; ++g) {
+          BatchTask t;
+          t.inst = i;
+          t.cell_begin = std::min(ce, cb + g * cpb);
+          t.cell_end = std::min(ce, cb + (g + 1) * cpb);
+          t.fix_begin = t.fix_end = 0;
+          if (col == 0) {
+            t.fix_begin = (int32_t)(op.n_constrained * g / groups);
+            t.fix_end = (int32_t)(op.n_constrained * (g + 1) / groups);
+          }
+          t.wait_slot = last_slot[i];
+          t.wait_count = last_count[i];
+          t.done_slot = slot;
+          tv.push_back(t);
+        }
+        last_slot[i] = slot;
+        last_count[i] = groups;
+      }
+      colour_tasks.push_back((int)tv.size());
+    }
+    n_tasks = (int)tv.size();
+    upload_vec(inst, hi, s);
+    upload_vec(tasks, tv, s);
+    counters.allocate((size_t)(2 + n_slots) * sizeof(int));
+    MFG_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes, s));
+    MFG_HIP(hipStreamSynchronize(s));
+  }
+
+  void build(const Tuning &tu, hipStream_t s) {
+    if (prec == MFG_FP64) build_t<double>(tu, s);
+    else build_t<float>(tu, s);
+  }
+
+  template <typename Number, int WAVES>
+  void launch_t(hipStream_t s, const void *vecs, const int *active, int fix_mode, int assign, int queue,
+                int *error) const {
+    BatchApplyArgs<Number> a;
+    a.inst = inst.as<BatchInst<Number>>();
+    a.tasks = tasks.as<BatchTask>();
+    a.vecs = (const BatchVecs<Number> *)vecs;
+    a.active = active;
+    a.queue = queue;
+    a.fix_mode = fix_mode;
+    a.assign = assign;
+    a.n_slots = n_slots;
+    a.counters = counters.as<int>();
+    a.error = error;
+    if (n_tasks == 0) return;
+    if (queue) {
+      a.task_begin = 0;
+      a.task_end = n_tasks;
+      const int grid = std::min(n_tasks, queue_capacity<Number, WAVES>());
+      batched_apply_kernel<Number, WAVES><<<grid, kBatchThreads, 0, s>>>(a);
+      MFG_HIP(hipGetLastError());
+      return;
+    }
+    for (size_t col = 0; col + 1 < colour_tasks.size(); ++col) {
+      a.task_begin = colour_tasks[col];
+      a.task_end = colour_tasks[col + 1];
+      if (a.task_end == a.task_begin) continue;
+      batched_apply_kernel<Number, WAVES><<<a.task_end - a.task_begin, kBatchThreads, 0, s>>>(a);
+      MFG_HIP(hipGetLastError());
+    }
+  }
+
+  // vmult (fix_mode 1, first-touch stores) or vmult_add (fix_mode 2, adds)
+  void launch(hipStream_t s, const void *vecs, const int *active, bool add, const Tuning &tu, int *error) const {
+    const int fix = add ? 2 : 1, assign = add ? 0 : 1;
+    if (prec == MFG_FP64) launch_t<double, 2>(s, vecs, active, fix, assign, tu.queue, error);
+    else if (tu.waves == 4) launch_t<float, 4>(s, vecs, active, fix, assign, tu.queue, error);
+    else launch_t<float, 2>(s, vecs, active, fix, assign, tu.queue, error);
+  }
+
+  // Table of (src, dst) per entry, in entry order.
+  void make_vecs(DeviceBuffer &buf, const std::vector<const void *> &src, const std::vector<void *> &dst,
+                 hipStream_t s) const {
+    std::vector<const void *> v(2 * ops.size());
+    for (size_t i = 0; i < ops.size(); ++i) {
+      MFG_REQUIRE(src[i] && dst[i], MFG_ERR_ARGUMENT, "null vector");
+      MFG_REQUIRE(src[i] != dst[i], MFG_ERR_ARGUMENT, "vmult needs distinct source and destination");
+      v[2 * i] = src[i];
+      v[2 * i + 1] = dst[i];
+    }
+    upload_vec(buf, v, s);
+  }
+
+  // Picks cells per task for every code of the group, then the schedule, by
+  // timing applications restricted (through the active mask) to the
+  // instances of one code.
+  void tune(Tuning &tu, const void *vecs, int reps, hipStream_t s, int *error) {
+    const int n = (int)ops.size();
+    int max_id = 0;
+    for (int id : ids) max_id = std::max(max_id, id);
+    DeviceBuffer mask((size_t)(max_id + 1) * sizeof(int));
+    std::vector<int> hm(max_id + 1, 0);
+    hipEvent_t e0, e1;
+    MFG_HIP(hipEventCreate(&e0));
+    MFG_HIP(hipEventCreate(&e1));
+    auto time_it = [&](const int *active, const Tuning &t) {
+      for (int w = 0; w < 2; ++w) launch(s, vecs, active, false, t, error);
+      MFG_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) launch(s, vecs, active, false, t, error);
+      MFG_HIP(hipEventRecord(e1, s));
+      MFG_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      MFG_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / reps;
+    };
+    try {
+      std::vector<int> codes;
+      for (int i = 0; i < n; ++i) {
+        const int c = code_of(*ops[i]);
+        if (std::find(codes.begin(), codes.end(), c) == codes.end()) codes.push_back(c);
+      }
+      for (int code : codes) {
+        std::fill(hm.begin(), hm.end(), 0);
+        for (int i = 0; i < n; ++i)
+          if (code_of(*ops[i]) == code) hm[ids[i]] = 1;
+        MFG_HIP(hipMemcpy(mask.ptr, hm.data(), hm.size() * sizeof(int), hipMemcpyHostToDevice));
+        std::vector<int> cand;
+        for (int c = max_cpb(code & 7); c >= 1; c /= 2) cand.push_back(c);
+        if (cand.size() > 4) cand.resize(4);
+        double best = 1e300;
+        int best_c = cand[0];
+        for (int c : cand) {
+          tu.cpb[code] = c;
+          build(tu, s);
+          const double ms = time_it(mask.as<int>(), tu);
+          if (ms < best) {
+            best = ms;
+            best_c = c;
+          }
+        }
+        tu.cpb[code] = best_c;
+      }
+      build(tu, s);
+      // register budget (FP32), then the schedule, on the whole batch
+      if (prec == MFG_FP32) {
+        Tuning t2 = tu, t4 = tu;
+        t2.waves = 2;
+        t4.waves = 4;
+        tu.waves = time_it(nullptr, t4) < time_it(nullptr, t2) ? 4 : 2;
+      }
+      Tuning tc = tu, tq = tu;
+      tc.queue = 0;
+      tq.queue = 1;
+      tu.ms_colour = time_it(nullptr, tc);
+      tu.ms_queue = time_it(nullptr, tq);
+      tu.queue = tu.ms_queue <= tu.ms_colour ? 1 : 0;
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    MFG_HIP(hipEventDestroy(e0));
+    MFG_HIP(hipEventDestroy(e1));
+  }
+};
+
+// -------------------------------------------------- segmented vectors ---
+struct SegTable {
+  DeviceBuffer segs;
+  int n = 0, prec = MFG_FP64, prec_a = MFG_FP64;
+  int64_t chunks = 0;
+  void build(std::vector<VecSeg> v, hipStream_t s) {
+    int64_t c = 0;
+    for (VecSeg &g : v) {
+      g.chunk_begin = c;
+      c += (g.n + kSegChunk - 1) / kSegChunk;
+    }
+    chunks = c;
+    n = (int)v.size();
+    upload_vec(segs, v, s);
+  }
+};
+
+template <typename T, typename TA, int OP>
+static void seg_launch_t(const SegTable &t, const int *active, const double *dyn, hipStream_t s) {
+  if (t.chunks == 0) return;
+  seg_kernel<T, TA, OP><<<(unsigned)t.chunks, kSegThreads, 0, s>>>(t.segs.as<VecSeg>(), t.n, active, dyn);
+  MFG_HIP(hipGetLastError());
+}
+
+template <int OP>
+static void seg_launch(const SegTable &t, const int *active, const double *dyn, hipStream_t s) {
+  if (OP == kSegConvert) {
+    if (t.prec == MFG_FP64 && t.prec_a == MFG_FP64) seg_launch_t<double, double, OP>(t, active, dyn, s);
+    else if (t.prec == MFG_FP64) seg_launch_t<double, float, OP>(t, active, dyn, s);
+    else if (t.prec_a == MFG_FP64) seg_launch_t<float, double, OP>(t, active, dyn, s);
+    else seg_launch_t<float, float, OP>(t, active, dyn, s);
+  } else if (t.prec == MFG_FP64) {
+    seg_launch_t<double, double, OP>(t, active, dyn, s);
+  } else {
+    seg_launch_t<float, float, OP>(t, active, dyn, s);
+  }
+}
+
+static VecSeg make_seg(void *y, const void *a, const void *b, const void *c, const void *d, int64_t n, int inst,
+                       double s0 = 0, double s1 = 0) {
+  VecSeg g;
+  std::memset(&g, 0, sizeof(g));
+  g.y = y;
+  g.a = a;
+  g.b = b;
+  g.c = c;
+  g.d = d;
+  g.n = n;
+  g.inst = inst;
+  g.s0 = s0;
+  g.s1 = s1;
+  return g;
+}
+
+// Per-instance sums with the single-vector reduction order; one table entry
+// per instance, in instance order.
+struct SegReducer {
+  DeviceBuffer partials, sums;
+  double *host = nullptr;
+  int n = 0;
+  void ensure(int ni) {
+    if (ni <= n) return;
+    partials.allocate((size_t)ni * kRedBlocks * sizeof(double));
+    sums.allocate((size_t)ni * sizeof(double));
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    MFG_HIP(hipHostMalloc((void **)&host, (size_t)ni * sizeof(double), hipHostMallocDefault));
+    n = ni;
+  }
+  ~SegReducer() {
+    if (host) (void)hipHostFree(host);
+  }
+  // launches only; fetch() reads the sums back
+  void dot(const SegTable &t, const int *active, hipStream_t s) {
+    ensure(t.n);
+    const dim3 grid(kRedBlocks, t.n);
+    if (t.prec == MFG_FP64)
+      seg_dot_partial_kernel<double><<<grid, kRedThreads, 0, s>>>(t.segs.as<VecSeg>(), active, partials.as<double>());
+    else
+      seg_dot_partial_kernel<float><<<grid, kRedThreads, 0, s>>>(t.segs.as<VecSeg>(), active, partials.as<double>());
+    seg_sum_partials_kernel<<<t.n, kRedThreads, 0, s>>>(partials.as<double>(), sums.as<double>());
+    MFG_HIP(hipGetLastError());
+  }
+  void cg_update(const SegTable &t, const int *active, const double *alpha, hipStream_t s) {
+    ensure(t.n);
+    const dim3 grid(kRedBlocks, t.n);
+    if (t.prec == MFG_FP64)
+      seg_cg_update_kernel<double><<<grid, kRedThreads, 0, s>>>(t.segs.as<VecSeg>(), active, alpha,
+                                                                partials.as<double>());
+    else
+      seg_cg_update_kernel<float><<<grid, kRedThreads, 0, s>>>(t.segs.as<VecSeg>(), active, alpha,
+                                                               partials.as<double>());
+    seg_sum_partials_kernel<<<t.n, kRedThreads, 0, s>>>(partials.as<double>(), sums.as<double>());
+    MFG_HIP(hipGetLastError());
+  }
+  std::vector<double> fetch(int ni, hipStream_t s) {
+    MFG_HIP(hipMemcpyAsync(host, sums.ptr, (size_t)ni * sizeof(double), hipMemcpyDeviceToHost, s));
+    MFG_HIP(hipStreamSynchronize(s));
+    return std::vector<double>(host, host + ni);
+  }
+};
+
+// --------------------------------------------------- batched transfers ---
+// Restriction (coarse colours in turn) or prolongation between one pair of
+// levels of many instances.
+struct TransferGroup {
+  int kind = 0;  // 0 prolongate_add, 1 restrict_add
+  int cprec = MFG_FP32, fprec = MFG_FP32;
+  DeviceBuffer tab, tasks;
+  std::vector<int> colour_tasks;
+
+  template <typename CN, typename FN>
+  void build_t(const std::vector<TransferData *> &tr, const std::vector<MGLevel *> &coarse,
+               const std::vector<MGLevel *> &fine, const std::vector<int> &ids, hipStream_t s) {
+    const int n = (int)tr.size();
+    std::vector<BatchTransfer<CN, FN>> h(n);
+    int n_col = 0;
+    for (int i = 0; i < n; ++i) {
+      const TransferData &t = *tr[i];
+      BatchTransfer<CN, FN> &e = h[i];
+      std::memset(&e, 0, sizeof(e));
+      e.args.coarse_nodes = t.coarse->cell_nodes.as<uint32_t>();
+      e.args.lattice = t.lattice.as<uint32_t>();
+      e.args.E = t.E.as<double>();
+      if (kind == 0) {
+        e.args.coarse_in = coarse[i]->sol.as<CN>();
+        e.args.fine_out = fine[i]->sol.as<FN>();
+      } else {
+        e.args.coarse_out = coarse[i]->def.as<CN>();
+        e.args.fine_in = fine[i]->tmp.as<FN>();
+      }
+      e.code = t.p + (t.nc == 3 ? 8 : 0);
+      e.inst = ids[i];
+      n_col = std::max(n_col, (int)t.coarse->colour_start.size() - 1);
+    }
+    std::vector<BatchCellTask> tv;
+    colour_tasks.assign(1, 0);
+    if (kind == 0) {  // every fine node has one owner: no colouring needed
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < tr[i]->coarse->n_cells; ++c) tv.push_back({i, c});
+      colour_tasks.push_back((int)tv.size());
+    } else {
+      for (int col = 0; col < n_col; ++col) {
+        for (int i = 0; i < n; ++i) {
+          const std::vector<int> &cs = tr[i]->coarse->colour_start;
+          if (col + 1 >= (int)cs.size()) continue;
+          for (int c = cs[col]; c < cs[col + 1]; ++c) tv.push_back({i, c});
+        }
+        colour_tasks.push_back((int)tv.size());
+      }
+    }
+    upload_vec(tab, h, s);
+    upload_vec(tasks, tv, s);
+  }
+
+  void build(int k, const std::vector<TransferData *> &tr, const std::vector<MGLevel *> &coarse,
+             const std::vector<MGLevel *> &fine, const std::vector<int> &ids, hipStream_t s) {
+    kind = k;
+    cprec = coarse[0]->op->precision;
+    fprec = fine[0]->op->precision;
+    for (size_t i = 0; i < tr.size(); ++i)
+      MFG_REQUIRE(coarse[i]->op->precision == cprec && fine[i]->op->precision == fprec, MFG_ERR_ARGUMENT,
+                  "batched levels must share one precision per level");
+    if (cprec == MFG_FP64 && fprec == MFG_FP64) build_t<double, double>(tr, coarse, fine, ids, s);
+    else if (cprec == MFG_FP64) build_t<double, float>(tr, coarse, fine, ids, s);
+    else if (fprec == MFG_FP64) build_t<float, double>(tr, coarse, fine, ids, s);
+    else build_t<float, float>(tr, coarse, fine, ids, s);
+  }
+
+  template <typename CN, typename FN>
+  void launch_t(hipStream_t s, const int *active) const {
+    for (size_t col = 0; col + 1 < colour_tasks.size(); ++col) {
+      const int b = colour_tasks[col], e = colour_tasks[col + 1];
+      if (e == b) continue;
+      if (kind == 0)
+        batched_transfer_kernel<0, CN, FN><<<e - b, 256, 0, s>>>(tab.as<BatchTransfer<CN, FN>>(),
+                                                                  tasks.as<BatchCellTask>(), b, e, active);
+      else
+        batched_transfer_kernel<1, CN, FN><<<e - b, 256, 0, s>>>(tab.as<BatchTransfer<CN, FN>>(),
+                                                                  tasks.as<BatchCellTask>(), b, e, active);
+      MFG_HIP(hipGetLastError());
+    }
+  }
+  void launch(hipStream_t s, const int *active) const {
+    if (cprec == MFG_FP64 && fprec == MFG_FP64) launch_t<double, double>(s, active);
+    else if (cprec == MFG_FP64) launch_t<double, float>(s, active);
+    else if (fprec == MFG_FP64) launch_t<float, double>(s, active);
+    else launch_t<float, float>(s, active);
+  }
+};
+
+// ---------------------------------------------------- batched vmult API ---
+struct BatchVmult {
+  std::unique_ptr<ApplyGroup> group;
+  Tuning tu;
+  hipStream_t stream = nullptr;
+  DeviceBuffer vecs, error;
+  std::vector<const void *> last_src;
+  std::vector<void *> last_dst;
+
+  BatchVmult(const std::vector<const OperatorData *> &ops) {
+    std::vector<int> ids(ops.size());
+    for (size_t i = 0; i < ids.size(); ++i) ids[i] = (int)i;
+    group.reset(new ApplyGroup(ops, ids));
+    stream = ops[0]->stream;
+    error.allocate(sizeof(int));
+    MFG_HIP(hipMemset(error.ptr, 0, sizeof(int)));
+    MFG_HIP(hipDeviceSynchronize());  // operator set-up on other streams is complete
+    group->build(tu, stream);
+  }
+  void set_vectors(const void *const *src, void *const *dst) {
+    const size_t n = group->ops.size();
+    std::vector<const void *> s(src, src + n);
+    std::vector<void *> d(dst, dst + n);
+    if (s == last_src && d == last_dst && vecs.ptr) return;
+    group->make_vecs(vecs, s, d, stream);
+    last_src = s;
+    last_dst = d;
+  }
+  void apply(const void *const *src, void *const *dst, bool add) {
+    set_vectors(src, dst);
+    group->launch(stream, vecs.ptr, nullptr, add, tu, error.as<int>());
+  }
+  void synchronize() {
+    MFG_HIP(hipStreamSynchronize(stream));
+    int e = 0;
+    MFG_HIP(hipMemcpy(&e, error.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    MFG_REQUIRE(e == 0, MFG_ERR_HIP, "batched kernel reported a wait timeout");
+  }
+};
+
+// --------------------------------------------------- batched MG and CG ---
+struct BatchSolver {
+  int n = 0, G = 0;
+  std::vector<OperatorData *> fine;
+  std::vector<MGData *> mg;
+  hipStream_t stream = nullptr;
+  Tuning tu_fine, tu_mg;
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;
+  size_t graph_nodes = 0;
+
+  std::unique_ptr<ApplyGroup> fine_group;
+  std::map<std::string, std::unique_ptr<ApplyGroup>> level_groups;
+  std::vector<std::unique_ptr<DeviceBuffer>> tables;
+  std::vector<std::unique_ptr<SegTable>> segs;
+  std::vector<std::unique_ptr<TransferGroup>> transfers;
+  std::vector<std::function<void(hipStream_t)>> program;
+  int program_launches = 0;
+
+  // outer-CG workspace (fine precision), one slice per instance
+  DeviceBuffer ws_r, ws_z, ws_p, ws_h;
+  std::vector<int64_t> off;
+  DeviceBuffer active, dyn, error;
+  int *host_active = nullptr;
+  double *host_dyn = nullptr;
+  SegReducer red;
+  // tables over the workspace built once
+  DeviceBuffer vec_hp;
+  SegTable seg_pz, seg_xpby, dot_rr, dot_rz, dot_ph;
+
+  int fprec() const { return fine[0]->precision; }
+  size_t fsize() const { return fine[0]->elem_size(); }
+  void *slice(DeviceBuffer &b, int i) const { return (char *)b.ptr + off[i] * fsize(); }
+
+  BatchSolver(const std::vector<OperatorData *> &f, const std::vector<MGData *> &m) : fine(f), mg(m) {
+    n = (int)fine.size();
+    MFG_REQUIRE(n > 0 && mg.size() == fine.size(), MFG_ERR_ARGUMENT, "empty batch");
+    for (int i = 0; i < n; ++i) {
+      MFG_REQUIRE(fine[i] && fine[i]->ready, MFG_ERR_STATE, "fine operator used before cache()");
+      MFG_REQUIRE(fine[i]->precision == fine[0]->precision, MFG_ERR_ARGUMENT,
+                  "fine operators must share one precision");
+      MFG_REQUIRE(mg[i] && mg[i]->ready, MFG_ERR_STATE, "multigrid not set up");
+      MFG_REQUIRE(mg[i]->coarse_kind() == 0, MFG_ERR_ARGUMENT,
+                  "the batched V-cycle needs the Chebyshev coarse solver (coarse_kind 0)");
+      MFG_REQUIRE(mg[i]->levels.back().op->n_dofs == fine[i]->n_dofs, MFG_ERR_ARGUMENT,
+                  "finest multigrid level and fine operator differ in size");
+      G = std::max(G, (int)mg[i]->levels.size());
+    }
+    MFG_HIP(hipDeviceSynchronize());
+    MFG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    off.resize(n + 1, 0);
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + ((fine[i]->n_dofs + 63) / 64) * 64;
+    for (DeviceBuffer *b : {&ws_r, &ws_z, &ws_p, &ws_h}) b->allocate((size_t)off[n] * fsize());
+    active.allocate((size_t)n * sizeof(int));
+    dyn.allocate((size_t)n * sizeof(double));
+    error.allocate(sizeof(int));
+    MFG_HIP(hipMemset(error.ptr, 0, sizeof(int)));
+    MFG_HIP(hipHostMalloc((void **)&host_active, (size_t)n * sizeof(int), hipHostMallocDefault));
+    MFG_HIP(hipHostMalloc((void **)&host_dyn, (size_t)n * sizeof(double), hipHostMallocDefault));
+    for (int i = 0; i < n; ++i) host_active[i] = 1;
+    upload_active();
+    red.ensure(n);
+    build();
+  }
+  ~BatchSolver() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    if (host_active) (void)hipHostFree(host_active);
+    if (host_dyn) (void)hipHostFree(host_dyn);
+  }
+
+  void upload_active() {
+    MFG_HIP(hipMemcpyAsync(active.ptr, host_active, (size_t)n * sizeof(int), hipMemcpyHostToDevice, stream));
+  }
+  void upload_dyn() {
+    MFG_HIP(hipMemcpyAsync(dyn.ptr, host_dyn, (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream));
+  }
+  void check_error() {
+    int e = 0;
+    MFG_HIP(hipMemcpy(&e, error.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    MFG_REQUIRE(e == 0, MFG_ERR_HIP, "batched kernel reported a wait timeout");
+  }
+
+  SegTable *new_seg(std::vector<VecSeg> v, int prec, int prec_a = -1) {
+    segs.emplace_back(new SegTable());
+    SegTable *t = segs.back().get();
+    t->prec = prec;
+    t->prec_a = prec_a < 0 ? prec : prec_a;
+    t->build(std::move(v), stream);
+    return t;
+  }
+  const void *new_table(const ApplyGroup &g, const std::vector<const void *> &src, const std::vector<void *> &dst) {
+    tables.emplace_back(new DeviceBuffer());
+    g.make_vecs(*tables.back(), src, dst, stream);
+    return tables.back()->ptr;
+  }
+
+  // apply group for the instances `ins` at global level g
+  ApplyGroup &level_group(int g, const std::vector<int> &ins) {
+    std::string key = std::to_string(g) + ":";
+    for (int i : ins) key += std::to_string(i) + ",";
+    auto it = level_groups.find(key);
+    if (it != level_groups.end()) return *it->second;
+    std::vector<const OperatorData *> ops;
+    for (int i : ins) ops.push_back(mg[i]->levels[g].op);
+    std::unique_ptr<ApplyGroup> grp(new ApplyGroup(ops, ins));
+    grp->build(tu_mg, stream);
+    ApplyGroup &ref = *grp;
+    level_groups[key] = std::move(grp);
+    return ref;
+  }
+
+  // t = A x at level g (x = sol, or given by `src_tmp`: tmp = A sol)
+  void add_apply(int g, const std::vector<int> &ins, bool to_tmp) {
+    if (ins.empty()) return;
+    // split by precision of the level operators
+    std::map<int, std::vector<int>> by_prec;
+    for (int i : ins) by_prec[mg[i]->levels[g].op->precision].push_back(i);
+    for (auto &kv : by_prec) {
+      ApplyGroup &grp = level_group(g, kv.second);
+      std::vector<const void *> src;
+      std::vector<void *> dst;
+      for (int i : kv.second) {
+        MGLevel &L = mg[i]->levels[g];
+        src.push_back(L.sol.ptr);
+        dst.push_back(to_tmp ? L.tmp.ptr : L.t.ptr);
+      }
+      const void *vt = new_table(grp, src, dst);
+      const ApplyGroup *gp = &grp;
+      program.push_back([this, gp, vt](hipStream_t s) {
+        gp->launch(s, vt, active.as<int>(), false, tu_mg, error.as<int>());
+      });
+      program_launches += tu_mg.queue ? 1 : (int)gp->colour_tasks.size() - 1;
+    }
+  }
+
+  template <int OP>
+  void add_seg_op(std::vector<std::pair<VecSeg, int>> items) {  // (segment, precision)
+    std::map<int, std::vector<VecSeg>> by_prec;
+    for (auto &it : items) by_prec[it.second].push_back(it.first);
+    for (auto &kv : by_prec) {
+      SegTable *t = new_seg(kv.second, kv.first);
+      program.push_back([this, t](hipStream_t s) { seg_launch<OP>(*t, active.as<int>(), dyn.as<double>(), s); });
+      ++program_launches;
+    }
+  }
+
+  // MGLevel::chebyshev for the instances `ins` at global level g, x = sol, b = def
+  void add_chebyshev(int g, const std::vector<int> &ins, bool zero_guess) {
+    std::vector<std::pair<VecSeg, int>> first;
+    for (int i : ins) {
+      MGLevel &L = mg[i]->levels[g];
+      const int pr = L.op->precision;
+      const double inv_theta = 1.0 / L.theta;
+      if (zero_guess)
+        first.push_back({make_seg(L.sol.ptr, L.d.ptr, L.def.ptr, L.op->inv_diag.ptr, nullptr, L.op->n_dofs, i,
+                                  inv_theta),
+                         pr});
+      else
+        first.push_back({make_seg(L.sol.ptr, L.d.ptr, L.def.ptr, L.t.ptr, L.op->inv_diag.ptr, L.op->n_dofs, i,
+                                  inv_theta),
+                         pr});
+    }
+    if (zero_guess) {
+      add_seg_op<kSegChebFirst>(first);
+    } else {
+      add_apply(g, ins, false);
+      add_seg_op<kSegChebFirstStep>(first);
+    }
+    // recurrence coefficients per instance, as MGLevel::chebyshev computes them
+    std::vector<int> q;
+    std::vector<std::vector<std::pair<double, double>>> coef;
+    int kmax = 0;
+    for (int i : ins) {
+      MGLevel &L = mg[i]->levels[g];
+      if (L.degree < 2 || std::abs(L.delta) < 1e-40) continue;
+      std::vector<std::pair<double, double>> c;
+      double rhok = L.delta / L.theta;
+      const double sigma = L.theta / L.delta;
+      for (int k = 0; k < L.degree - 1; ++k) {
+        const double rhokp = 1.0 / (2.0 * sigma - rhok);
+        c.push_back({rhokp * rhok, 2.0 * rhokp / L.delta});
+        rhok = rhokp;
+      }
+      kmax = std::max(kmax, (int)c.size());
+      q.push_back(i);
+      coef.push_back(c);
+    }
+    for (int k = 0; k < kmax; ++k) {
+      std::vector<int> qk;
+      std::vector<std::pair<VecSeg, int>> upd;
+      for (size_t j = 0; j < q.size(); ++j) {
+        if (k >= (int)coef[j].size()) continue;
+        const int i = q[j];
+        MGLevel &L = mg[i]->levels[g];
+        qk.push_back(i);
+        upd.push_back({make_seg(L.sol.ptr, L.d.ptr, L.def.ptr, L.t.ptr, L.op->inv_diag.ptr, L.op->n_dofs, i,
+                                coef[j][k].first, coef[j][k].second),
+                       L.op->precision});
+      }
+      add_apply(g, qk, false);
+      add_seg_op<kSegChebUpdate>(upd);
+    }
+  }
+
+  void add_transfer(int g, const std::vector<int> &ins, int kind) {
+    std::vector<TransferData *> tr;
+    std::vector<MGLevel *> c, f;
+    for (int i : ins) {
+      tr.push_back(mg[i]->transfers[g - 1]);
+      c.push_back(&mg[i]->levels[g - 1]);
+      f.push_back(&mg[i]->levels[g]);
+    }
+    transfers.emplace_back(new TransferGroup());
+    TransferGroup *t = transfers.back().get();
+    t->build(kind, tr, c, f, ins, stream);
+    program.push_back([this, t](hipStream_t s) { t->launch(s, active.as<int>()); });
+    program_launches += (int)t->colour_tasks.size() - 1;
+  }
+
+  // The V-cycle of every instance (MGData::vmult from r to z), bottom-aligned:
+  // global level g is level g of every instance that has it.
+  void build_vcycle() {
+    program.clear();
+    program_launches = 0;
+    level_groups.clear();
+    tables.clear();
+    segs.clear();
+    transfers.clear();
+    auto has = [&](int g) {
+      std::vector<int> v;
+      for (int i = 0; i < n; ++i)
+        if ((int)mg[i]->levels.size() > g) v.push_back(i);
+      return v;
+    };
+    {  // copy_to_mg: def of the finest level = r
+      std::map<int, std::vector<VecSeg>> by;
+      for (int i = 0; i < n; ++i) {
+        MGLevel &T = mg[i]->levels.back();
+        by[T.op->precision].push_back(make_seg(T.def.ptr, slice(ws_r, i), nullptr, nullptr, nullptr, T.op->n_dofs, i));
+      }
+      for (auto &kv : by) {
+        SegTable *t = new_seg(kv.second, kv.first, fprec());
+        program.push_back([this, t](hipStream_t s) { seg_launch<kSegConvert>(*t, active.as<int>(), nullptr, s); });
+        ++program_launches;
+      }
+    }
+    for (int g = G - 1; g >= 1; --g) {  // down: pre-smooth, residual, restrict
+      const std::vector<int> P = has(g);
+      add_chebyshev(g, P, true);
+      add_apply(g, P, true);
+      std::vector<std::pair<VecSeg, int>> res, fill;
+      for (int i : P) {
+        MGLevel &L = mg[i]->levels[g], &C = mg[i]->levels[g - 1];
+        res.push_back({make_seg(L.tmp.ptr, L.def.ptr, nullptr, nullptr, nullptr, L.op->n_dofs, i), L.op->precision});
+        fill.push_back({make_seg(C.def.ptr, nullptr, nullptr, nullptr, nullptr, C.op->n_dofs, i, 0.0),
+                        C.op->precision});
+      }
+      add_seg_op<kSegResidual>(res);
+      add_seg_op<kSegFill>(fill);
+      add_transfer(g, P, 1);
+    }
+    add_chebyshev(0, has(0), true);  // coarse: Chebyshev solver mode
+    for (int g = 1; g < G; ++g) {  // up: prolongate, post-smooth
+      const std::vector<int> P = has(g);
+      add_transfer(g, P, 0);
+      add_chebyshev(g, P, false);
+    }
+    {  // copy_from_mg: z = sol of the finest level
+      std::map<int, std::vector<VecSeg>> by;
+      for (int i = 0; i < n; ++i) {
+        MGLevel &T = mg[i]->levels.back();
+        by[T.op->precision].push_back(make_seg(slice(ws_z, i), T.sol.ptr, nullptr, nullptr, nullptr, T.op->n_dofs, i));
+      }
+      for (auto &kv : by) {
+        SegTable *t = new_seg(kv.second, fprec(), kv.first);
+        program.push_back([this, t](hipStream_t s) { seg_launch<kSegConvert>(*t, active.as<int>(), nullptr, s); });
+        ++program_launches;
+      }
+    }
+    if (graph) {
+      MFG_HIP(hipGraphExecDestroy(graph));
+      graph = nullptr;
+    }
+  }
+
+  void build() {
+    std::vector<const OperatorData *> f(fine.begin(), fine.end());
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i) ids[i] = i;
+    fine_group.reset(new ApplyGroup(f, ids));
+    fine_group->build(tu_fine, stream);
+    std::vector<const void *> src;
+    std::vector<void *> dst;
+    std::vector<VecSeg> pz, xp, rr, rz, ph;
+    for (int i = 0; i < n; ++i) {
+      const int64_t m = fine[i]->n_dofs;
+      src.push_back(slice(ws_p, i));
+      dst.push_back(slice(ws_h, i));
+      pz.push_back(make_seg(slice(ws_p, i), slice(ws_z, i), nullptr, nullptr, nullptr, m, i));
+      xp.push_back(make_seg(slice(ws_p, i), slice(ws_z, i), nullptr, nullptr, nullptr, m, i));
+      rr.push_back(make_seg(nullptr, slice(ws_r, i), slice(ws_r, i), nullptr, nullptr, m, i));
+      rz.push_back(make_seg(nullptr, slice(ws_r, i), slice(ws_z, i), nullptr, nullptr, m, i));
+      ph.push_back(make_seg(nullptr, slice(ws_p, i), slice(ws_h, i), nullptr, nullptr, m, i));
+    }
+    fine_group->make_vecs(vec_hp, src, dst, stream);
+    for (auto &pr : std::vector<std::pair<SegTable *, std::vector<VecSeg> *>>{
+             {&seg_pz, &pz}, {&seg_xpby, &xp}, {&dot_rr, &rr}, {&dot_rz, &rz}, {&dot_ph, &ph}}) {
+      pr.first->prec = pr.first->prec_a = fprec();
+      pr.first->build(*pr.second, stream);
+    }
+    build_vcycle();
+  }
+
+  void run_vcycle() {
+    if (!use_graph) {
+      for (auto &f : program) f(stream);
+      return;
+    }
+    if (!graph) {
+      hipGraph_t g = nullptr;
+      MFG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        for (auto &f : program) f(stream);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      MFG_HIP(hipStreamEndCapture(stream, &g));
+      size_t nn = 0;
+      (void)hipGraphGetNodes(g, nullptr, &nn);
+      graph_nodes = nn;
+      const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      MFG_HIP(e);
+    }
+    MFG_HIP(hipGraphLaunch(graph, stream));
+  }
+
+  void tune(int reps) {
+    MFG_HIP(hipDeviceSynchronize());
+    // fine operators on the workspace vectors
+    fine_group->tune(tu_fine, vec_hp.ptr, reps, stream, error.as<int>());
+    // level operators: the finest level of every instance (the bulk of the V-cycle)
+    std::vector<const OperatorData *> ops;
+    std::vector<int> ids;
+    std::vector<const void *> src;
+    std::vector<void *> dst;
+    for (int i = 0; i < n; ++i) {
+      MGLevel &T = mg[i]->levels.back();
+      if (!ops.empty() && T.op->precision != ops[0]->precision) continue;
+      ops.push_back(T.op);
+      ids.push_back(i);
+      src.push_back(T.sol.ptr);
+      dst.push_back(T.t.ptr);
+    }
+    ApplyGroup top(ops, ids);
+    top.build(tu_mg, stream);
+    DeviceBuffer vt;
+    top.make_vecs(vt, src, dst, stream);
+    top.tune(tu_mg, vt.ptr, reps, stream, error.as<int>());
+    fine_group->build(tu_fine, stream);
+    build_vcycle();
+    MFG_HIP(hipStreamSynchronize(stream));
+    check_error();
+  }
+
+  // z = MG(r) for every instance
+  void precondition(const void *const *src, void *const *dst) {
+    for (int i = 0; i < n; ++i) host_active[i] = 1;
+    upload_active();
+    std::vector<VecSeg> in, out;
+    for (int i = 0; i < n; ++i) {
+      in.push_back(make_seg(slice(ws_r, i), src[i], nullptr, nullptr, nullptr, fine[i]->n_dofs, i));
+      out.push_back(make_seg(dst[i], slice(ws_z, i), nullptr, nullptr, nullptr, fine[i]->n_dofs, i));
+    }
+    SegTable ti, to;
+    ti.prec = ti.prec_a = to.prec = to.prec_a = fprec();
+    ti.build(in, stream);
+    to.build(out, stream);
+    seg_launch<kSegConvert>(ti, nullptr, nullptr, stream);
+    run_vcycle();
+    seg_launch<kSegConvert>(to, nullptr, nullptr, stream);
+    MFG_HIP(hipStreamSynchronize(stream));
+    check_error();
+  }
+
+  // cg_solve with the multigrid preconditioner, all instances in lockstep
+  void solve(const void *const *b, void *const *x, double tol_rel, int max_iter, mfg_solve_info *info,
+             int32_t *status) {
+    for (int i = 0; i < n; ++i) {
+      MFG_REQUIRE(b[i] && x[i], MFG_ERR_ARGUMENT, "null vector");
+      host_active[i] = 1;
+    }
+    upload_active();
+    // per-solve tables over the caller's vectors
+    DeviceBuffer vec_hx;
+    std::vector<const void *> src(x, x + n);
+    std::vector<void *> dst;
+    std::vector<VecSeg> rb, bb, upd;
+    for (int i = 0; i < n; ++i) {
+      const int64_t m = fine[i]->n_dofs;
+      dst.push_back(slice(ws_h, i));
+      rb.push_back(make_seg(slice(ws_r, i), b[i], slice(ws_h, i), nullptr, nullptr, m, i, -1.0));
+      bb.push_back(make_seg(nullptr, b[i], b[i], nullptr, nullptr, m, i));
+      upd.push_back(make_seg(x[i], slice(ws_r, i), slice(ws_p, i), slice(ws_h, i), nullptr, m, i));
+    }
+    fine_group->make_vecs(vec_hx, src, dst, stream);
+    SegTable t_rb, t_bb, t_upd;
+    for (SegTable *t : {&t_rb, &t_bb, &t_upd}) t->prec = t->prec_a = fprec();
+    t_rb.build(rb, stream);
+    t_bb.build(bb, stream);
+    t_upd.build(upd, stream);
+    const int *act = active.as<int>();
+    // r = b - A x
+    fine_group->launch(stream, vec_hx.ptr, act, false, tu_fine, error.as<int>());
+    seg_launch<kSegAddScaled>(t_rb, act, nullptr, stream);
+    red.dot(t_bb, act, stream);
+    const std::vector<double> bn = red.fetch(n, stream);
+    red.dot(dot_rr, act, stream);
+    const std::vector<double> r0 = red.fetch(n, stream);
+    std::vector<double> tol(n), res(n), gh(n);
+    std::vector<std::vector<double>> alphas(n), betas(n);
+    int n_active = 0;
+    for (int i = 0; i < n; ++i) {
+      tol[i] = tol_rel * std::sqrt(bn[i]);
+      res[i] = std::sqrt(r0[i]);
+      info[i].initial_residual = res[i];
+      info[i].iterations = 0;
+      info[i].condition_estimate = 1.0;
+      status[i] = MFG_OK;
+      host_active[i] = res[i] > tol[i] ? 1 : 0;
+      n_active += host_active[i];
+    }
+    if (n_active) {
+      upload_active();
+      run_vcycle();
+      red.dot(dot_rz, act, stream);
+      const std::vector<double> g0 = red.fetch(n, stream);
+      for (int i = 0; i < n; ++i) gh[i] = g0[i];
+      seg_launch<kSegConvert>(seg_pz, act, nullptr, stream);
+      for (int it = 1; it <= max_iter && n_active; ++it) {
+        fine_group->launch(stream, vec_hp.ptr, act, false, tu_fine, error.as<int>());
+        red.dot(dot_ph, act, stream);
+        const std::vector<double> ph = red.fetch(n, stream);
+        bool changed = false;
+        for (int i = 0; i < n; ++i) {
+          if (!host_active[i]) continue;
+          if (!(ph[i] > 0)) {  // breakdown of this instance
+            status[i] = MFG_ERR_NUMERIC;
+            host_active[i] = 0;
+            --n_active;
+            changed = true;
+            continue;
+          }
+          host_dyn[i] = gh[i] / ph[i];
+          alphas[i].push_back(host_dyn[i]);
+        }
+        if (changed) upload_active();
+        if (!n_active) break;
+        upload_dyn();
+        red.cg_update(t_upd, act, dyn.as<double>(), stream);
+        const std::vector<double> rr = red.fetch(n, stream);
+        changed = false;
+        for (int i = 0; i < n; ++i) {
+          if (!host_active[i]) continue;
+          res[i] = std::sqrt(rr[i]);
+          info[i].iterations = it;
+          if (res[i] <= tol[i]) {
+            host_active[i] = 0;
+            --n_active;
+            changed = true;
+          }
+        }
+        if (!n_active) break;
+        if (changed) upload_active();
+        run_vcycle();
+        red.dot(dot_rz, act, stream);
+        const std::vector<double> ghn = red.fetch(n, stream);
+        for (int i = 0; i < n; ++i) {
+          if (!host_active[i]) continue;
+          host_dyn[i] = ghn[i] / gh[i];
+          gh[i] = ghn[i];
+          betas[i].push_back(host_dyn[i]);
+        }
+        upload_dyn();
+        seg_launch<kSegXpby>(seg_xpby, act, dyn.as<double>(), stream);
+      }
+    }
+    MFG_HIP(hipStreamSynchronize(stream));
+    check_error();
+    for (int i = 0; i < n; ++i) {
+      info[i].final_residual = res[i];
+      if (!alphas[i].empty()) {
+        std::vector<double> &al = alphas[i];
+        if (al.size() > betas[i].size() + 1) al.resize(betas[i].size() + 1);
+        double lmin = 0, lmax = 0;
+        lanczos_estimate(al, betas[i], lmin, lmax);
+        info[i].condition_estimate = lmin > 0 ? lmax / lmin : 0.0;
+      }
+      if (status[i] == MFG_OK && res[i] > tol[i]) status[i] = MFG_ERR_NOCONV;
+    }
+  }
+};
+
+}  // namespace mfg
+
+using namespace mfg;
+
+struct mfg_batch_t {
+  std::unique_ptr<BatchVmult> b;
+};
+struct mfg_batch_solver_t {
+  std::unique_ptr<BatchSolver> s;
+};
+
+static void export_tuning(const Tuning &tu, mfg_batch_tuning *out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof(*out));
+  out->schedule = tu.queue ? MFG_SCHEDULE_QUEUE : MFG_SCHEDULE_COLOUR;
+  for (int kind = 0; kind < 3; ++kind)
+    for (int p = 1; p <= kMaxDegree; ++p) out->cells_per_task[kind][p] = tu.cpb_for(kind * 8 + p);
+  out->waves_per_simd = tu.waves;
+  out->ms_colour = tu.ms_colour;
+  out->ms_queue = tu.ms_queue;
+}
+
+extern "C" {
+
+int mfg_batch_create(int32_t n, const mfg_operator *ops, mfg_batch *out) {
+  return guarded([&] {
+    MFG_REQUIRE(out && ops && n > 0, MFG_ERR_ARGUMENT, "null argument");
+    std::vector<const OperatorData *> v;
+    for (int i = 0; i < n; ++i) {
+      MFG_REQUIRE(ops[i], MFG_ERR_ARGUMENT, "null operator");
+      v.push_back(ops[i]);
+    }
+    std::unique_ptr<mfg_batch_t> b(new mfg_batch_t());
+    b->b.reset(new BatchVmult(v));
+    *out = b.release();
+  });
+}
+
+int mfg_batch_destroy(mfg_batch b) {
+  return guarded([&] { delete b; });
+}
+
+int mfg_batch_set_stream(mfg_batch b, void *stream) {
+  return guarded([&] {
+    MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
+    b->b->stream = (hipStream_t)stream;
+  });
+}
+
+int mfg_batch_set_schedule(mfg_batch b, int32_t schedule) {
+  return guarded([&] {
+    MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
+    MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE, MFG_ERR_ARGUMENT,
+                "unknown schedule");
+    b->b->tu.queue = schedule == MFG_SCHEDULE_QUEUE;
+  });
+}
+
+int mfg_batch_set_cells_per_task(mfg_batch b, int32_t kind, int32_t degree, int32_t cells) {
+  return guarded([&] {
+    MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
+    MFG_REQUIRE(kind >= 0 && kind < 3 && degree >= 1 && degree <= kMaxDegree, MFG_ERR_ARGUMENT,
+                "unknown kind or degree");
+    MFG_REQUIRE(cells >= 0 && cells <= max_cpb(degree), MFG_ERR_ARGUMENT, "cells per task out of range");
+    b->b->tu.cpb[kind * 8 + degree] = cells;
+    b->b->group->build(b->b->tu, b->b->stream);
+  });
+}
+
+int mfg_batch_vmult(mfg_batch b, void *const *dst, const void *const *src) {
+  return guarded([&] {
+    MFG_REQUIRE(b && dst && src, MFG_ERR_ARGUMENT, "null argument");
+    b->b->apply(src, dst, false);
+  });
+}
+
+int mfg_batch_vmult_add(mfg_batch b, void *const *dst, const void *const *src) {
+  return guarded([&] {
+    MFG_REQUIRE(b && dst && src, MFG_ERR_ARGUMENT, "null argument");
+    b->b->apply(src, dst, true);
+  });
+}
+
+int mfg_batch_synchronize(mfg_batch b) {
+  return guarded([&] {
+    MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
+    b->b->synchronize();
+  });
+}
+
+int mfg_batch_tune(mfg_batch b, const void *const *src, void *const *dst, int32_t repetitions,
+                   mfg_batch_tuning *result) {
+  return guarded([&] {
+    MFG_REQUIRE(b && src && dst && repetitions > 0, MFG_ERR_ARGUMENT, "null argument");
+    BatchVmult &v = *b->b;
+    v.set_vectors(src, dst);
+    v.group->tune(v.tu, v.vecs.ptr, repetitions, v.stream, v.error.as<int>());
+    v.synchronize();
+    export_tuning(v.tu, result);
+  });
+}
+
+int mfg_batch_get_tuning(mfg_batch b, mfg_batch_tuning *result) {
+  return guarded([&] {
+    MFG_REQUIRE(b && result, MFG_ERR_ARGUMENT, "null argument");
+    export_tuning(b->b->tu, result);
+  });
+}
+
+int mfg_batch_solver_create(int32_t n, const mfg_operator *fine, const mfg_mg *mgs, mfg_batch_solver *out) {
+  return guarded([&] {
+    MFG_REQUIRE(out && fine && mgs && n > 0, MFG_ERR_ARGUMENT, "null argument");
+    std::vector<OperatorData *> f;
+    std::vector<MGData *> m;
+    for (int i = 0; i < n; ++i) {
+      MFG_REQUIRE(fine[i] && mgs[i], MFG_ERR_ARGUMENT, "null operator or multigrid");
+      f.push_back(fine[i]);
+      m.push_back(mgs[i]);
+    }
+    std::unique_ptr<mfg_batch_solver_t> s(new mfg_batch_solver_t());
+    s->s.reset(new BatchSolver(f, m));
+    *out = s.release();
+  });
+}
+
+int mfg_batch_solver_destroy(mfg_batch_solver s) {
+  return guarded([&] { delete s; });
+}
+
+int mfg_batch_solver_use_graph(mfg_batch_solver s, int32_t on) {
+  return guarded([&] {
+    MFG_REQUIRE(s, MFG_ERR_ARGUMENT, "null solver");
+    s->s->use_graph = on != 0;
+  });
+}
+
+int mfg_batch_solver_tune(mfg_batch_solver s, int32_t repetitions, mfg_batch_tuning *fine,
+                          mfg_batch_tuning *levels) {
+  return guarded([&] {
+    MFG_REQUIRE(s && repetitions > 0, MFG_ERR_ARGUMENT, "null argument");
+    s->s->tune(repetitions);
+    export_tuning(s->s->tu_fine, fine);
+    export_tuning(s->s->tu_mg, levels);
+  });
+}
+
+int mfg_batch_solver_set_schedule(mfg_batch_solver s, int32_t schedule) {
+  return guarded([&] {
+    MFG_REQUIRE(s, MFG_ERR_ARGUMENT, "null solver");
+    MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE, MFG_ERR_ARGUMENT,
+                "unknown schedule");
+    s->s->tu_fine.queue = s->s->tu_mg.queue = schedule == MFG_SCHEDULE_QUEUE;
+    s->s->build();
+  });
+}
+
+int mfg_batch_solver_precondition(mfg_batch_solver s, void *const *dst, const void *const *src) {
+  return guarded([&] {
+    MFG_REQUIRE(s && dst && src, MFG_ERR_ARGUMENT, "null argument");
+    s->s->precondition(src, dst);
+  });
+}
+
+int mfg_batch_solver_solve(mfg_batch_solver s, const void *const *b, void *const *x, double tol_rel,
+                           int32_t max_iter, mfg_solve_info *info, int32_t *status) {
+  return guarded([&] {
+    MFG_REQUIRE(s && b && x && info && status, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(tol_rel > 0 && max_iter > 0, MFG_ERR_ARGUMENT, "invalid tolerance or iteration limit");
+    s->s->solve(b, x, tol_rel, max_iter, info, status);
+  });
+}
+
+int mfg_batch_solver_info(mfg_batch_solver s, int32_t *vcycle_launches, int32_t *graph_nodes) {
+  return guarded([&] {
+    MFG_REQUIRE(s, MFG_ERR_ARGUMENT, "null solver");
+    if (vcycle_launches) *vcycle_launches = s->s->program_launches;
+    if (graph_nodes) *graph_nodes = (int32_t)s->s->graph_nodes;
+  });
+}
+
+}  // extern "C"

@@ -360,6 +360,60 @@ struct ApplyArgs {
   int fix_mode;
 };
 
+// The cells [cell_begin, min(cell_begin + cpb, cell_end)) on all threads of
+// the workgroup, thread t owning point t % N3 of local cell t / N3; threads
+// past cpb*N3 only join the barriers.  `assign`: the first colour touching a
+// node stores instead of adding.  Shape and collocation tables must be in LDS
+// before the first barrier inside.
+template <int P, typename Number, class QOp>
+__device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ cell_nodes,
+                                  const Number *__restrict__ src, Number *__restrict__ dst, int cell_begin,
+                                  int cell_end, int cpb, int assign, const Number *s_shape, const Number *s_colloc,
+                                  Number *s_tile) {
+  constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC;
+  const int t = threadIdx.x;
+  const int lc_raw = t / N3;
+  const bool pad = lc_raw >= cpb;
+  const int lc = pad ? cpb - 1 : lc_raw;
+  const int pt = pad ? 0 : t - lc_raw * N3;
+  Tile<N, Number> T;
+  T.base = s_tile + lc * 3 * NC * N3;
+  T.pt = pt;
+  T.i = pt % N;
+  T.j = (pt / N) % N;
+  T.k = pt / (N * N);
+  T.writer = !pad;
+  const int cell = cell_begin + lc;
+  const bool active = !pad && cell < cell_end;
+  const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
+
+  uint32_t entry = active ? cell_nodes[qidx] : 0u;
+  const int64_t node = entry & kNodeMaskDev;
+  Number u[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const bool skip = !active || ((entry >> (kConstraintShift + c)) & 1u);
+    u[c] = skip ? Number(0) : src[node * NC + c];
+  }
+  const typename QOp::Data qd = qop.load(qidx);
+  __syncthreads();
+
+  Number val[NC], grad[NC][3];
+  evaluate_vg<N, NC, Number>(T, s_shape, s_colloc, u, val, grad);
+  qop.apply(qd, val, grad);
+  Number out[NC];
+  integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
+  if (active) {
+    const bool first = assign && (entry >> 31);  // first colour touching the node stores
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (!((entry >> (kConstraintShift + c)) & 1u)) {
+        if (first) dst[node * NC + c] = out[c];
+        else dst[node * NC + c] += out[c];
+      }
+  }
+}
+
 template <int P, int CPB, typename Number, class QOp>
 __global__ void __launch_bounds__((block_threads<P, CPB>()))
     apply_cells(ApplyArgs<Number> a, QOp qop, int assign) {
@@ -380,46 +434,9 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
       else a.dst[dof] += a.src[dof];
     }
   }
-  const int lc_raw = t / N3;
-  const bool pad = lc_raw >= CPB;
-  const int lc = pad ? CPB - 1 : lc_raw;
-  const int pt = pad ? 0 : t - lc_raw * N3;
-  Tile<N, Number> T;
-  T.base = s_tile + lc * 3 * NC * N3;
-  T.pt = pt;
-  T.i = pt % N;
-  T.j = (pt / N) % N;
-  T.k = pt / (N * N);
-  T.writer = !pad;
-  const int cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB + lc;
-  const bool active = !pad && cell < a.cell_end;
-  const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
-
-  uint32_t entry = active ? a.cell_nodes[qidx] : 0u;
-  const int64_t node = entry & kNodeMaskDev;
-  Number u[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const bool skip = !active || ((entry >> (kConstraintShift + c)) & 1u);
-    u[c] = skip ? Number(0) : a.src[node * NC + c];
-  }
-  const typename QOp::Data qd = qop.load(qidx);
-  __syncthreads();
-
-  Number val[NC], grad[NC][3];
-  evaluate_vg<N, NC, Number>(T, s_shape, s_colloc, u, val, grad);
-  qop.apply(qd, val, grad);
-  Number out[NC];
-  integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
-  if (active) {
-    const bool first = assign && (entry >> 31);  // first colour touching the node stores
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-      if (!((entry >> (kConstraintShift + c)) & 1u)) {
-        if (first) a.dst[node * NC + c] = out[c];
-        else a.dst[node * NC + c] += out[c];
-      }
-  }
+  const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
+  cell_group<P, Number, QOp>(qop, a.cell_nodes, a.src, a.dst, first_cell, a.cell_end, CPB, assign, s_shape,
+                             s_colloc, s_tile);
 }
 
 }  // namespace mfg

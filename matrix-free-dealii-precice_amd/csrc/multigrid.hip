@@ -281,7 +281,8 @@ static void tridiag_eigs(const std::vector<double> &dg, const std::vector<double
   lmax = kth(n - 1);
 }
 
-static void lanczos(const std::vector<double> &alphas, const std::vector<double> &betas, double &lmin, double &lmax) {
+void lanczos_estimate(const std::vector<double> &alphas, const std::vector<double> &betas, double &lmin,
+                      double &lmax) {
   const int k = (int)alphas.size();
   std::vector<double> dg(k), off(k > 0 ? k - 1 : 0);
   for (int i = 0; i < k; ++i) {
@@ -340,7 +341,7 @@ void MGLevel::estimate_eigenvalues(int n_it, Reducer &red) {
     vec_xpby(pr, cg_p.ptr, cg_z.ptr, beta, n, s);
   }
   if (alphas.size() > betas.size() + 1) alphas.resize(betas.size() + 1);
-  lanczos(alphas, betas, lmin, lmax);
+  lanczos_estimate(alphas, betas, lmin, lmax);
 }
 
 // [1.2 lmax / range, 1.2 lmax]; range < 1: solver mode with the degree from
@@ -411,7 +412,11 @@ void MGData::setup() {
   for (size_t l = 0; l < levels.size(); ++l) {
     MGLevel &L = levels[l];
     if (l == 0) {
-      if (coarse_kind() == 0) L.setup_chebyshev(0, prm.coarse_range, (int)L.op->n_dofs, red);
+      if (coarse_kind() == 0) {
+        int64_t it = L.op->n_dofs;
+        if (prm.coarse_eig_iterations > 0) it = std::min<int64_t>(it, prm.coarse_eig_iterations);
+        L.setup_chebyshev(0, prm.coarse_range, (int)it, red);
+      }
       else L.setup_chebyshev(prm.cheb_degree, prm.smoothing_range, prm.eig_cg_iterations, red);
     } else {
       L.setup_chebyshev(prm.cheb_degree, prm.smoothing_range, prm.eig_cg_iterations, red);
@@ -533,7 +538,7 @@ void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, cons
   if (!alphas.empty()) {
     if (alphas.size() > betas.size() + 1) alphas.resize(betas.size() + 1);
     double lmin, lmax;
-    lanczos(alphas, betas, lmin, lmax);
+    lanczos_estimate(alphas, betas, lmin, lmax);
     info->condition_estimate = lmin > 0 ? lmax / lmin : 0.0;
   }
   if (res > tol) throw Error(MFG_ERR_NOCONV, "CG did not converge within the iteration limit");

@@ -72,6 +72,10 @@ struct CGWorkspace {
   void ensure(size_t bytes);
 };
 
+// Extreme eigenvalues of the Lanczos matrix of CG coefficients.
+void lanczos_estimate(const std::vector<double> &alphas, const std::vector<double> &betas, double &lmin,
+                      double &lmax);
+
 void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, const void *b, void *x, double tol_rel,
               int max_iter, mfg_solve_info *info, CGWorkspace &ws, Reducer &red);
 

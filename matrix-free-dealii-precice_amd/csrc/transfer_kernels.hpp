@@ -34,15 +34,19 @@ struct TransferArgs {
   FN *fine_out;
 };
 
-// Sweep helper: out[o] = sum_t M[r(o)][t] in[...]; generic strided loops over
-// the block's threads (sizes are small and vary with p).
+// LDS doubles one coarse cell needs (the largest of the three transfers)
+template <int P>
+constexpr int transfer_lds() {
+  constexpr int N = P + 1, M = 2 * P + 1;
+  return M * N + M * M * M + M * M * N + M * N * N;
+}
+
+// The per-cell bodies run on all threads of a workgroup (strided loops; the
+// sizes are small and vary with p) and use `lds` as scratch.
 template <int P, int NC, typename CN, typename FN>
-__global__ void __launch_bounds__(256) prolongate_add_kernel(TransferArgs<CN, FN> a) {
+__device__ inline void prolongate_cell(const TransferArgs<CN, FN> &a, int cell, double *lds) {
   constexpr int N = P + 1, M = 2 * P + 1, N3 = N * N * N, M3 = M * M * M;
-  __shared__ double sE[M * N];
-  __shared__ double su[N3], t1[M * N * N], t2[M * M * N];
-  const int cell = a.cell_begin + blockIdx.x;
-  if (cell >= a.cell_end) return;
+  double *sE = lds, *su = sE + M * N, *t1 = su + N3, *t2 = t1 + M * N * N;
   const int t = threadIdx.x, nt = blockDim.x;
   for (int x = t; x < M * N; x += nt) sE[x] = a.E[x];
   for (int c = 0; c < NC; ++c) {
@@ -68,7 +72,7 @@ __global__ void __launch_bounds__(256) prolongate_add_kernel(TransferArgs<CN, FN
     }
     __syncthreads();
     for (int x = t; x < M3; x += nt) {
-      const uint32_t e = a.lattice[(int64_t)(cell - 0) * M3 + x];
+      const uint32_t e = a.lattice[(int64_t)cell * M3 + x];
       if (!(e & kOwnerBit) || ((e >> (kConstraintShift + c)) & 1u)) continue;
       const int IJ = x % (M * M), K = x / (M * M);
       double s = 0;
@@ -79,14 +83,11 @@ __global__ void __launch_bounds__(256) prolongate_add_kernel(TransferArgs<CN, FN
   }
 }
 
-// coarse += P^T fine, for the coarse cells of one colour (no shared coarse node)
+// coarse += P^T fine on one coarse cell (callers run one colour at a time)
 template <int P, int NC, typename CN, typename FN>
-__global__ void __launch_bounds__(256) restrict_add_kernel(TransferArgs<CN, FN> a) {
+__device__ inline void restrict_cell(const TransferArgs<CN, FN> &a, int cell, double *lds) {
   constexpr int N = P + 1, M = 2 * P + 1, N3 = N * N * N, M3 = M * M * M;
-  __shared__ double sE[M * N];
-  __shared__ double sf[M3], t2[M * M * N], t1[M * N * N];
-  const int cell = a.cell_begin + blockIdx.x;
-  if (cell >= a.cell_end) return;
+  double *sE = lds, *sf = sE + M * N, *t2 = sf + M3, *t1 = t2 + M * M * N;
   const int t = threadIdx.x, nt = blockDim.x;
   for (int x = t; x < M * N; x += nt) sE[x] = a.E[x];
   for (int c = 0; c < NC; ++c) {
@@ -127,12 +128,9 @@ __global__ void __launch_bounds__(256) restrict_add_kernel(TransferArgs<CN, FN> 
 // interpolation matrix (p+1) x (2p+1).  Shared coarse nodes receive
 // bit-identical values from every cell containing them.
 template <int P, int NC, typename CN, typename FN>
-__global__ void __launch_bounds__(256) interpolate_kernel(TransferArgs<CN, FN> a) {
+__device__ inline void interpolate_cell(const TransferArgs<CN, FN> &a, int cell, double *lds) {
   constexpr int N = P + 1, M = 2 * P + 1, N3 = N * N * N, M3 = M * M * M;
-  __shared__ double sI[N * M];
-  __shared__ double sf[M3], t2[M * M * N], t1[M * N * N];
-  const int cell = a.cell_begin + blockIdx.x;
-  if (cell >= a.cell_end) return;
+  double *sI = lds, *sf = sI + N * M, *t2 = sf + M3, *t1 = t2 + M * M * N;
   const int t = threadIdx.x, nt = blockDim.x;
   for (int x = t; x < N * M; x += nt) sI[x] = a.E[x];
   for (int c = 0; c < NC; ++c) {
@@ -164,6 +162,26 @@ __global__ void __launch_bounds__(256) interpolate_kernel(TransferArgs<CN, FN> a
       a.coarse_out[(int64_t)(e & kNodeMaskDev) * NC + c] = (CN)s;
     }
   }
+}
+
+// One workgroup of 256 threads per coarse cell of [cell_begin, cell_end).
+template <int P, int NC, typename CN, typename FN>
+__global__ void __launch_bounds__(256) prolongate_add_kernel(TransferArgs<CN, FN> a) {
+  __shared__ double lds[transfer_lds<P>()];
+  const int cell = a.cell_begin + blockIdx.x;
+  if (cell < a.cell_end) prolongate_cell<P, NC, CN, FN>(a, cell, lds);
+}
+template <int P, int NC, typename CN, typename FN>
+__global__ void __launch_bounds__(256) restrict_add_kernel(TransferArgs<CN, FN> a) {
+  __shared__ double lds[transfer_lds<P>()];
+  const int cell = a.cell_begin + blockIdx.x;
+  if (cell < a.cell_end) restrict_cell<P, NC, CN, FN>(a, cell, lds);
+}
+template <int P, int NC, typename CN, typename FN>
+__global__ void __launch_bounds__(256) interpolate_kernel(TransferArgs<CN, FN> a) {
+  __shared__ double lds[transfer_lds<P>()];
+  const int cell = a.cell_begin + blockIdx.x;
+  if (cell < a.cell_end) interpolate_cell<P, NC, CN, FN>(a, cell, lds);
 }
 
 }  // namespace mfg

@@ -315,7 +315,8 @@ PRECOND_NONE, PRECOND_JACOBI, PRECOND_MG = 0, 1, 2
 class MGParams(ctypes.Structure):
     _fields_ = [("cheb_degree", ctypes.c_int32), ("smoothing_range", ctypes.c_double),
                 ("eig_cg_iterations", ctypes.c_int32), ("coarse_kind", ctypes.c_int32),
-                ("coarse_range", ctypes.c_double), ("coarse_tolerance", ctypes.c_double)]
+                ("coarse_range", ctypes.c_double), ("coarse_tolerance", ctypes.c_double),
+                ("coarse_eig_iterations", ctypes.c_int32)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -360,12 +361,12 @@ class Multigrid:
     """Multigrid + PreconditionMG with Chebyshev smoothing (mf_elasticity.h:1140-1279)."""
 
     def __init__(self, ops, transfers, cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0,
-                 coarse_range=1e-3, coarse_tolerance=1e-3):
+                 coarse_range=1e-3, coarse_tolerance=1e-3, coarse_eig_iterations=0):
         L = len(ops)
         oa = (_vp * L)(*[o.h for o in ops])
         ta = (_vp * max(L - 1, 1))(*([t.h for t in transfers] or [None]))
         self.prm = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, coarse_range,
-                            coarse_tolerance)
+                            coarse_tolerance, coarse_eig_iterations)
         h = _vp()
         check(lib().mfg_mg_create(L, oa, ta, ctypes.byref(self.prm), ctypes.byref(h)))
         self.h, self._keep = h, (list(ops), list(transfers))
@@ -558,3 +559,188 @@ class SolidProblem:
 
     def memory(self):
         return lib().mfg_solid_problem_memory(self.h)
+
+
+# ---------------------------------------------------- batched instances ---
+SCHEDULE_COLOUR, SCHEDULE_QUEUE = 0, 1
+BATCH_SOLID_F1, BATCH_SOLID_F0, BATCH_HEAT = 0, 1, 2
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("schedule", ctypes.c_int32), ("cells_per_task", (ctypes.c_int32 * 7) * 3),
+                ("waves_per_simd", ctypes.c_int32), ("ms_colour", ctypes.c_double), ("ms_queue", ctypes.c_double)]
+
+    def as_dict(self):
+        kinds = ("solid_f1", "solid_f0", "heat")
+        return {"schedule": "queue" if self.schedule == SCHEDULE_QUEUE else "colour",
+                "cells_per_task": {k: list(self.cells_per_task[i])[1:] for i, k in enumerate(kinds)},
+                "waves_per_simd": self.waves_per_simd, "ms_colour": self.ms_colour, "ms_queue": self.ms_queue}
+
+
+_vpp = ctypes.POINTER(_vp)
+SIGNATURES.update({
+    "mfg_batch_create": (ctypes.c_int, [ctypes.c_int32, _vpp, _vpp]),
+    "mfg_batch_destroy": (ctypes.c_int, [_vp]),
+    "mfg_batch_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "mfg_batch_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_batch_set_cells_per_task": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "mfg_batch_vmult": (ctypes.c_int, [_vp, _vpp, _vpp]),
+    "mfg_batch_vmult_add": (ctypes.c_int, [_vp, _vpp, _vpp]),
+    "mfg_batch_synchronize": (ctypes.c_int, [_vp]),
+    "mfg_batch_tune": (ctypes.c_int, [_vp, _vpp, _vpp, ctypes.c_int32, ctypes.POINTER(BatchTuning)]),
+    "mfg_batch_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BatchTuning)]),
+    "mfg_batch_solver_create": (ctypes.c_int, [ctypes.c_int32, _vpp, _vpp, _vpp]),
+    "mfg_batch_solver_destroy": (ctypes.c_int, [_vp]),
+    "mfg_batch_solver_use_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_batch_solver_set_schedule": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_batch_solver_tune": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(BatchTuning),
+                                             ctypes.POINTER(BatchTuning)]),
+    "mfg_batch_solver_precondition": (ctypes.c_int, [_vp, _vpp, _vpp]),
+    "mfg_batch_solver_solve": (ctypes.c_int, [_vp, _vpp, _vpp, ctypes.c_double, ctypes.c_int32,
+                                              ctypes.POINTER(SolveInfo), _c_i32p]),
+    "mfg_batch_solver_info": (ctypes.c_int, [_vp, _c_i32p, _c_i32p]),
+})
+
+
+def _ptr_array(ts):
+    return (_vp * len(ts))(*[_ptr(t) for t in ts])
+
+
+class Batch:
+    """Many operators (one precision) applied in one launch per vmult."""
+
+    def __init__(self, ops):
+        arr = (_vp * len(ops))(*[o.h for o in ops])
+        h = _vp()
+        check(lib().mfg_batch_create(len(ops), arr, ctypes.byref(h)))
+        self.h, self._keep, self.n = h, list(ops), len(ops)
+        self._destroy = lib().mfg_batch_destroy
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def set_schedule(self, schedule):
+        check(lib().mfg_batch_set_schedule(self.h, schedule))
+
+    def set_cells_per_task(self, kind, degree, cells):
+        check(lib().mfg_batch_set_cells_per_task(self.h, kind, degree, cells))
+
+    def vmult(self, dsts, srcs):
+        check(lib().mfg_batch_vmult(self.h, _ptr_array(dsts), _ptr_array(srcs)))
+
+    def vmult_add(self, dsts, srcs):
+        check(lib().mfg_batch_vmult_add(self.h, _ptr_array(dsts), _ptr_array(srcs)))
+
+    def synchronize(self):
+        check(lib().mfg_batch_synchronize(self.h))
+
+    def tune(self, srcs, dsts, repetitions=5):
+        t = BatchTuning()
+        check(lib().mfg_batch_tune(self.h, _ptr_array(srcs), _ptr_array(dsts), repetitions, ctypes.byref(t)))
+        return t
+
+    def tuning(self):
+        t = BatchTuning()
+        check(lib().mfg_batch_get_tuning(self.h, ctypes.byref(t)))
+        return t
+
+
+class BatchSolver:
+    """MG-preconditioned CG on many instances in lockstep; the V-cycle of all
+    instances is one program of batched launches replayed as a hipGraph."""
+
+    def __init__(self, fine_ops, mgs):
+        n = len(fine_ops)
+        fa = (_vp * n)(*[o.h for o in fine_ops])
+        ma = (_vp * n)(*[m.h for m in mgs])
+        h = _vp()
+        check(lib().mfg_batch_solver_create(n, fa, ma, ctypes.byref(h)))
+        self.h, self._keep, self.n = h, (list(fine_ops), list(mgs)), n
+        self._destroy = lib().mfg_batch_solver_destroy
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._destroy(self.h)
+            self.h = None
+
+    def use_graph(self, on=True):
+        check(lib().mfg_batch_solver_use_graph(self.h, 1 if on else 0))
+
+    def set_schedule(self, schedule):
+        check(lib().mfg_batch_solver_set_schedule(self.h, schedule))
+
+    def tune(self, repetitions=5):
+        f, l = BatchTuning(), BatchTuning()
+        check(lib().mfg_batch_solver_tune(self.h, repetitions, ctypes.byref(f), ctypes.byref(l)))
+        return f, l
+
+    def precondition(self, dsts, srcs):
+        check(lib().mfg_batch_solver_precondition(self.h, _ptr_array(dsts), _ptr_array(srcs)))
+
+    def solve(self, bs, xs, tol_rel=1e-8, max_iter=1000):
+        """Returns (list of SolveInfo, list of status codes)."""
+        infos = (SolveInfo * self.n)()
+        status = (ctypes.c_int32 * self.n)()
+        check(lib().mfg_batch_solver_solve(self.h, _ptr_array(bs), _ptr_array(xs), tol_rel, max_iter, infos,
+                                           status))
+        return list(infos), list(status)
+
+    def info(self):
+        a, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib().mfg_batch_solver_info(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+class Instance:
+    """One problem of a batch on [lo, hi]: the fine operator in fine_precision
+    and a geometric MG hierarchy in level_precision (levels halve the cell
+    counts while all stay even).  Solid: neo-Hookean tensor2 linearised at
+    u0 = amp sin sin sin + noise (levels at the interpolated u), Dirichlet on
+    x = lo.  Heat: M + dt c K with constant conductivity c, Dirichlet on all
+    faces."""
+
+    def __init__(self, kind, n_cells, degree, fine_precision=FP64, level_precision=FP32, lo=(0.0, 0.0, 0.0),
+                 hi=(1.0, 1.0, 1.0), dt=0.1, conductivity=1.0, formulation=1, cheb_degree=5, smoothing_range=None,
+                 eig_cg_iterations=None, coarse_eig_iterations=0, amp=1e-3, noise=1e-5, seed=0,
+                 material=(384615.38461538462, 0.3, 1000.0, 4e6)):
+        import numpy as np
+        import torch
+        self.kind, self.degree = kind, degree
+        counts = [list(n_cells)]
+        while all(c % 2 == 0 for c in counts[0]):
+            counts.insert(0, [c // 2 for c in counts[0]])
+        solid = kind == "solid"
+        nc, fm, cm = (3, 1, 7) if solid else (1, 63, 1)
+        self.meshes = [BoxMesh(lo, hi, c, degree, nc, fm, cm) for c in counts]
+        fine_mesh = self.meshes[-1]
+        mu, nu, rho, alpha_1 = material
+        if solid:
+            make = lambda m, pr: NeoHookOperator(m, mu, nu, rho, alpha_1, formulation=formulation, precision=pr)
+        else:
+            def make(m, pr):
+                coef = np.full(m.n_cells * (degree + 1) ** 3, conductivity)
+                return LaplaceOperator(m, dt, coef, precision=pr)
+        self.levels = [make(m, level_precision) for m in self.meshes]
+        self.fine = make(fine_mesh, fine_precision)
+        self.transfers = [Transfer(self.levels[l - 1], self.levels[l], box_children(self.meshes[l - 1], self.meshes[l]))
+                          for l in range(1, len(self.meshes))]
+        if solid:
+            u = fine_mesh.u0(amp=amp, noise=noise, seed=seed)
+            self.fine.cache(torch.from_numpy(u).to("cuda", _dtype_of(fine_precision)))
+            ug = torch.from_numpy(u).to("cuda", _dtype_of(level_precision))
+            us = [ug]
+            for l in range(len(self.meshes) - 1, 0, -1):
+                c = torch.empty(self.meshes[l - 1].n_dofs, dtype=ug.dtype, device="cuda")
+                self.transfers[l - 1].interpolate(c, us[0])
+                us.insert(0, c)
+            for op, uu in zip(self.levels, us):
+                op.cache(uu)
+        sr = smoothing_range if smoothing_range is not None else (20.0 if solid else 15.0)
+        ei = eig_cg_iterations if eig_cg_iterations is not None else (30 if solid else 10)
+        self.mg = Multigrid(self.levels, self.transfers, cheb_degree=cheb_degree, smoothing_range=sr,
+                            eig_cg_iterations=ei, coarse_kind=0, coarse_eig_iterations=coarse_eig_iterations)
+        self.mg.setup()
+        self.n_dofs = fine_mesh.n_dofs
+        self.constrained = fine_mesh.constrained()
