@@ -26,6 +26,11 @@ HBM_PEAK_GBS = 8000.0
 BEAM = dict(lo=[0.0, 0.0, 0.0], hi=[4.0, 1.0, 1.0], cells=[128, 32, 32], degree=4)
 
 
+def log(msg):
+    """Progress on stderr (stdout carries the one JSON line)."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def canonical_bytes(n_dof, n_cell, p, s=8, words_per_qp=18):
     """s*2*N_dof (src read + dst write) + 4*N_cell*(p+1)^3 (int32 map) + s*18*N_qp."""
     nq = n_cell * (p + 1) ** 3
@@ -109,6 +114,157 @@ def mgcg_bench(mfgpu, torch, cells, p, level_precision, x_true_seed=2):
             "err_vs_x_true": err}, x
 
 
+def batched_instances(mfgpu, n_inst=48, seed_base=3, coarse_eig_iterations=100):
+    """Instance k (seed 3+k): k < n/2 heat (M + dt c K, dt 0.1, c ~ U[0.5, 2], u = 0 on the boundary),
+    else solid (neo-Hookean tensor2, u = 0 on x = 0); p ~ U{1..6}, n_q = p+1; unit cube with
+    n_x, n_y, n_z ~ U{4..24}.  FP64 fine operator, FP32 MG (levels halve while all counts are even;
+    Chebyshev degree 5 on levels > 0, Chebyshev solver on level 0)."""
+    import numpy as np
+    out = []
+    for k in range(n_inst):
+        r = np.random.default_rng(seed_base + k)
+        kind = "heat" if k < n_inst // 2 else "solid"
+        p = int(r.integers(1, 7))
+        n = [int(x) for x in r.integers(4, 25, size=3)]
+        c = float(r.uniform(0.5, 2.0))
+        out.append(mfgpu.Instance(kind, n, p, fine_precision=mfgpu.FP64, level_precision=mfgpu.FP32, dt=0.1,
+                                  conductivity=c, coarse_eig_iterations=coarse_eig_iterations, seed=seed_base + k))
+    return out
+
+
+def batched_bench(mfgpu, torch, reps=20):
+    """Batched vmult (one launch for all 48 FP64 fine operators) and batched
+    MG-CG (FP64 outer CG, FP32 V-cycle as a replayed hipGraph), each against
+    the same work done instance by instance on the same GPU."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    inst = batched_instances(mfgpu)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    log(f"batched: {len(inst)} instances set up in {t_setup:.1f} s")
+    n_dofs = sum(i.n_dofs for i in inst)
+    srcs = [mfgpu.fill_uniform(500 + k, torch.empty(i.n_dofs, dtype=torch.float64, device=dev))
+            for k, i in enumerate(inst)]
+    dsts = [torch.empty_like(s) for s in srcs]
+    fine = [i.fine for i in inst]
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n
+
+    t_seq_vmult = timed(lambda: [op.vmult(d, s) for op, d, s in zip(fine, dsts, srcs)], reps)
+    batch = mfgpu.Batch(fine)
+    t_tune0 = time.perf_counter()
+    tuning = batch.tune(srcs, dsts, repetitions=5).as_dict()
+    t_tune = time.perf_counter() - t_tune0
+    t_batch_vmult = timed(lambda: batch.vmult(dsts, srcs), reps)
+    log(f"batched: vmult {t_batch_vmult * 1e3:.3f} ms batched, {t_seq_vmult * 1e3:.3f} ms sequential")
+    # canonical bytes: src + dst, int32 map, 18 (solid) / 11 (heat) words per quadrature point
+    B = sum(8 * 2 * i.n_dofs + (4 + 8 * (18 if i.kind == "solid" else 11)) * i.fine.n_qp for i in inst)
+    # MG-CG: rhs = A x_true (seeds 600+k, constrained entries zeroed), ||r|| <= 1e-8 ||b||
+    bs, xts = [], []
+    for k, i in enumerate(inst):
+        xt = mfgpu.fill_uniform(600 + k, torch.empty(i.n_dofs, dtype=torch.float64, device=dev))
+        xt[torch.from_numpy(i.constrained).long().to(dev)] = 0.0
+        b = torch.empty_like(xt)
+        i.fine.vmult(b, xt)
+        bs.append(b)
+        xts.append(xt)
+    solver = mfgpu.BatchSolver(fine, [i.mg for i in inst])
+    fine_t, level_t = solver.tune(repetitions=3)
+    xs = [torch.zeros_like(b) for b in bs]
+    log("batched: solver tuned")
+    solver.solve(bs, xs, tol_rel=1e-8)  # first call captures the graph
+    log("batched: first solve done")
+    xs = [torch.zeros_like(b) for b in bs]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    infos, status = solver.solve(bs, xs, tol_rel=1e-8)
+    torch.cuda.synchronize()
+    t_batch_solve = time.perf_counter() - t
+    log(f"batched: solve {t_batch_solve:.3f} s")
+    launches, nodes = solver.info()
+    xq = [torch.zeros_like(b) for b in bs]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    seq_its = []
+    for i, b, x in zip(inst, bs, xq):
+        seq_its.append(mfgpu.cg_solve(i.fine, b, x, mfgpu.PRECOND_MG, i.mg, tol_rel=1e-8).iterations)
+    torch.cuda.synchronize()
+    t_seq_solve = time.perf_counter() - t
+    same = all(torch.equal(a, b) for a, b in zip(xs, xq))
+    err = max(float(torch.linalg.norm(x - xt) / torch.linalg.norm(xt)) for x, xt in zip(xs, xts))
+    its = [i.iterations for i in infos]
+    return {
+        "workload": "48 instances (seed 3+k): 24 heat (M + 0.1 c K, c ~ U[0.5,2]) + 24 neo-Hookean solids, "
+                    "p ~ U{1..6}, unit cube n ~ U{4..24}^3, FP64 fine operators, FP32 MG",
+        "n_instances": len(inst), "n_dofs_total": int(n_dofs), "setup_s": round(t_setup, 2),
+        "levels": [len(i.meshes) for i in inst],
+        "vmult": {"batched_ms": round(t_batch_vmult * 1e3, 4), "sequential_ms": round(t_seq_vmult * 1e3, 4),
+                  "speedup": round(t_seq_vmult / t_batch_vmult, 3),
+                  "batched_gdofs": round(n_dofs / t_batch_vmult / 1e9, 3),
+                  "batched_canonical_gbs": round(B / t_batch_vmult / 1e9, 1), "tuning": tuning,
+                  "tune_s": round(t_tune, 2)},
+        "mgcg": {"batched_s": round(t_batch_solve, 4), "sequential_s": round(t_seq_solve, 4),
+                 "speedup": round(t_seq_solve / t_batch_solve, 3), "iterations_max": max(its),
+                 "iterations_mean": round(float(np.mean(its)), 2), "all_converged": all(s == 0 for s in status),
+                 "batched_equals_sequential": same, "max_err_vs_x_true": err,
+                 "vcycle_launches": launches, "graph_nodes": nodes,
+                 "tuning_fine": fine_t.as_dict(), "tuning_levels": level_t.as_dict()},
+    }
+
+
+def serving_bench(mfgpu, n_sessions=32, n_requests=64, rate=40.0, seed=100, max_concurrent=4):
+    """32 live solid sessions (p ~ U{2..4}, beam [0,4]x[0,1]^2 with n_x ~ U{8,12,..,48}, n_y = n_z = n_x/4,
+    config-0 material, u0 = 1e-3 sin sin sin + 1e-5 noise); requests are one Newmark step each (Newton to
+    ||R||/||R0|| <= 1e-8, MG-CG per linear solve) arriving as a Poisson process (seed 100).  Served with
+    continuous batching (mixed-precision MG, up to 4 steps in flight, HBM-budget admission) against the same
+    requests run one after another with plain FP64 MG."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for k in range(n_sessions):
+        p = int(rng.integers(2, 5))
+        nx = 4 * int(rng.integers(2, 13))
+        shapes.append((p, nx))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    sessions = rng.integers(0, n_sessions, n_requests)
+    reqs = [(int(s), float(t)) for s, t in zip(sessions, arrivals)]
+
+    def specs(level_precision):
+        return [mfgpu.session_spec([0.0, 0.0, 0.0], [4.0, 1.0, 1.0], [nx, nx // 4, nx // 4], p, MU, NU, RHO,
+                                   beta=BETA, dt=DT, level_precision=level_precision, tol_f=1e-8,
+                                   seed=seed + 1 + k)
+                for k, (p, nx) in enumerate(shapes)]
+
+    res, st, _ = mfgpu.serve(specs(mfgpu.FP32), reqs, max_concurrent=max_concurrent, real_time=True)
+    log(f"serving: {st.completed} steps, {st.steps_per_s:.1f} steps/s, p50 {st.latency_p50_s * 1e3:.1f} ms")
+    res64, st64, _ = mfgpu.serve(specs(mfgpu.FP64), reqs, max_concurrent=1, real_time=False)
+    log(f"serving: sequential FP64 {st64.steps_per_s:.1f} steps/s")
+    return {
+        "workload": f"{n_sessions} solid sessions (p ~ U{{2..4}}, beam n_x ~ U{{8..48}} step 4, n_y = n_z = n_x/4), "
+                    f"{n_requests} Newmark-step requests, Poisson rate {rate}/s (seed {seed})",
+        "n_dofs": [int(3 * (nx * p + 1) * ((nx // 4) * p + 1) ** 2) for p, nx in shapes],
+        "served": {"steps_per_s": round(st.steps_per_s, 2), "latency_p50_ms": round(st.latency_p50_s * 1e3, 2),
+                   "latency_p99_ms": round(st.latency_p99_s * 1e3, 2), "makespan_s": round(st.makespan_s, 3),
+                   "completed": st.completed, "failed": st.failed, "admissions": st.admissions,
+                   "evictions": st.evictions, "max_resident": st.max_resident,
+                   "peak_bytes": int(st.peak_bytes), "max_concurrent": max_concurrent,
+                   "level_precision": "fp32 (fp64 coarse, outer CG and Newton)"},
+        "sequential_fp64": {"steps_per_s": round(st64.steps_per_s, 2), "makespan_s": round(st64.makespan_s, 3),
+                            "latency_p50_ms": round(st64.latency_p50_s * 1e3, 2), "failed": st64.failed},
+        "throughput_vs_sequential_fp64": round(st.steps_per_s / max(st64.steps_per_s, 1e-30), 3),
+        "newton_iterations_mean": round(float(np.mean([r.newton_iterations for r in res])), 2),
+        "linear_iterations_mean": round(float(np.mean([r.linear_iterations for r in res])), 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +272,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true", help="skip the MG-CG legs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances leg")
+    ap.add_argument("--only-batched", action="store_true", help="run the batched-instances leg alone")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
     ap.add_argument("--degree", type=int, default=BEAM["degree"])
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
     args = ap.parse_args()
@@ -130,6 +290,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     dev = torch.device("cuda:0")
+    if args.only_batched:
+        print(json.dumps({"batched": batched_bench(mfgpu, torch)}), flush=True)
+        return
+    if args.only_serving:
+        print(json.dumps({"serving": serving_bench(mfgpu)}), flush=True)
+        return
     p = args.degree
     mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], args.cells, p, 3, 1, 7)
     n_dof, n_cell = mesh.n_dofs, mesh.n_cells
@@ -202,6 +368,12 @@ def main():
         out["mgcg"] = {"workload": "MG-CG to ||r||/||b|| <= 1e-8, rhs = A x_true (seed 2), Chebyshev degree 5, "
                                    "coarse FP64 CG to 1e-3", "fp64": r64, "mixed_fp32_vcycle": r32,
                        "speedup_mixed_vs_fp64": round(r64["solve_s"] / r32["solve_s"], 3)}
+    if not args.no_batched:
+        torch.cuda.empty_cache()
+        out["batched"] = batched_bench(mfgpu, torch)
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        out["serving"] = serving_bench(mfgpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
