@@ -8,9 +8,9 @@
 // directions into one LDS round trip.  Gather is a coalesced read of the
 // int32 cell->node map followed by per-node vector loads.  Cells come in
 // colours (no two cells of a colour share a node), one launch per colour, so
-// the scatter is a plain read-modify-write; the first colour that touches a
-// node stores instead (bit 31 of its map entry), which makes vmult need no
-// zero fill.  Constrained dofs are skipped via the constraint bits stored
+// the scatter is a plain read-modify-write, whose read is issued with the
+// gather; the first colour that touches a node stores instead (bit 31 of its
+// map entry), which makes vmult need no zero fill.  Constrained dofs are skipped via the constraint bits stored
 // above bit 28 of every map entry.
 //
 // Algorithmic reference: FEEvaluation::evaluate/integrate with
@@ -168,6 +168,14 @@ __device__ inline void evaluate_g(const Tile<N, Number> &T, const Number *S, con
   evaluate_vg<N, NC, Number>(T, S, D, u, val, grad);
 }
 
+// The quadrature data is read once per operator application: load it
+// non-temporally so that it does not evict from the caches the vector entries
+// that the next colour reads again (nodes shared between cells).
+template <typename Number>
+__device__ inline Number stream_load(const Number *p) {
+  return __builtin_nontemporal_load(p);
+}
+
 // ------------------------------------------------------ quadrature ops ---
 // Each op provides load(qidx) -> Data and apply(Data, val, grad), turning
 // values/reference gradients at a point into submitted value and reference
@@ -189,12 +197,12 @@ struct QSolidT2F1 {
   __device__ Data load(int64_t q) const {
     Data d;
 #pragma unroll
-    for (int f = 0; f < 9; ++f) d.ji[f] = qd[f * nqp + q];
-    d.jxw = qd[9 * nqp + q];
-    d.c2 = qd[10 * nqp + q];
-    d.c1 = qd[11 * nqp + q];
+    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
+    d.jxw = stream_load(qd + 9 * nqp + q);
+    d.c2 = stream_load(qd + 10 * nqp + q);
+    d.c1 = stream_load(qd + 11 * nqp + q);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) d.T[f] = qd[(12 + f) * nqp + q];
+    for (int f = 0; f < 6; ++f) d.T[f] = stream_load(qd + (12 + f) * nqp + q);
     return d;
   }
   __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
@@ -240,11 +248,11 @@ struct QSolidT2F0 {
   __device__ Data load(int64_t q) const {
     Data d;
 #pragma unroll
-    for (int f = 0; f < 9; ++f) d.ji[f] = qd[f * nqp + q];
-    d.jxw = qd[9 * nqp + q];
-    d.jm13 = qd[10 * nqp + q];
+    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
+    d.jxw = stream_load(qd + 9 * nqp + q);
+    d.jm13 = stream_load(qd + 10 * nqp + q);
 #pragma unroll
-    for (int f = 0; f < 9; ++f) d.F[f] = qd[(11 + f) * nqp + q];
+    for (int f = 0; f < 9; ++f) d.F[f] = stream_load(qd + (11 + f) * nqp + q);
     return d;
   }
   __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
@@ -326,9 +334,9 @@ struct QHeat {
   __device__ Data load(int64_t q) const {
     Data d;
 #pragma unroll
-    for (int f = 0; f < 9; ++f) d.ji[f] = qd[f * nqp + q];
-    d.jxw = qd[9 * nqp + q];
-    d.c = qd[10 * nqp + q];
+    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
+    d.jxw = stream_load(qd + 9 * nqp + q);
+    d.c = stream_load(qd + 10 * nqp + q);
     return d;
   }
   __device__ void apply(const Data &d, Number (&val)[1], Number (&grad)[1][3]) const {
@@ -395,6 +403,16 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
     const bool skip = !active || ((entry >> (kConstraintShift + c)) & 1u);
     u[c] = skip ? Number(0) : src[node * NC + c];
   }
+  // dst of this node, read now: no other workgroup of the launch writes it,
+  // so the scatter at the end is a plain store and this load's latency is
+  // hidden behind the sweeps.  The first colour touching a node stores.
+  const bool first = assign && (entry >> 31);
+  Number dold[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const bool skip = !active || first || ((entry >> (kConstraintShift + c)) & 1u);
+    dold[c] = skip ? Number(0) : dst[node * NC + c];
+  }
   const typename QOp::Data qd = qop.load(qidx);
   __syncthreads();
 
@@ -404,13 +422,9 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   Number out[NC];
   integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
   if (active) {
-    const bool first = assign && (entry >> 31);  // first colour touching the node stores
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      if (!((entry >> (kConstraintShift + c)) & 1u)) {
-        if (first) dst[node * NC + c] = out[c];
-        else dst[node * NC + c] += out[c];
-      }
+      if (!((entry >> (kConstraintShift + c)) & 1u)) dst[node * NC + c] = dold[c] + out[c];
   }
 }
 
