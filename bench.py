@@ -25,6 +25,11 @@ HBM_PEAK_GBS = 8000.0
 
 BEAM = dict(lo=[0.0, 0.0, 0.0], hi=[4.0, 1.0, 1.0], cells=[128, 32, 32], degree=4)
 
+# HBM bytes per vmult of the current cell kernel from two rocprofv3 --pmc passes (FETCH_SIZE x 2.00 and
+# WRITE_SIZE x 1.00, factors from a copy of known size); PMC counters cannot be read from inside this
+# process, so the figure is re-measured whenever the kernel changes.
+MEASURED_TRAFFIC = {((128, 32, 32), 4): 4017419059}
+
 
 def log(msg):
     """Progress on stderr (stdout carries the one JSON line)."""
@@ -352,7 +357,9 @@ def main():
                                "linearised at 1e-3 sin sin sin + 1e-5 noise",
                    "n_dofs": n_dof, "n_cells": n_cell, "degree": p, "caching": "tensor2", "formulation": 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p)),
+                     "traffic_source": "profiles/vmult_q4_beam_fp64_pmc_traffic.md",
                      "kernel": "apply_cells<4,2,double,QSolidT2F1>", "kernel_avg_ms": round(kern_avg_ms, 4),
                      "canonical_bytes_per_launch": B},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
