@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--only-batched", action="store_true", help="run the batched-instances leg alone")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter probe: 2 vmults, then 2 copies of a 1 GiB buffer (known bytes), no timing")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
     ap.add_argument("--degree", type=int, default=BEAM["degree"])
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
@@ -310,6 +312,17 @@ def main():
     del u0
     src = mfgpu.fill_uniform(1, torch.empty(n_dof, dtype=torch.float64, device=dev))
     dst = torch.empty_like(src)
+
+    if args.pmc_probe:
+        for _ in range(2):
+            op.vmult(dst, src)
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("pmc probe done")
+        return
 
     for _ in range(args.warmup):
         op.vmult(dst, src)

@@ -37,8 +37,15 @@ extern "C" {
 #define MFG_FP64 0
 #define MFG_FP32 1
 
-/* MF caching strategies (parameter_handling.h "MF caching"); the device path
- * implements tensor2, the strategy whose data the canonical roofline counts. */
+/* MF caching strategies (parameter_handling.h "MF caching",
+ * mf_nh_operator.h:346-523, 848-1010).  All four define the same operator:
+ *   scalar_referential: reference geometry + 1 scalar per point; F rebuilt
+ *                       from the cached nodal displacement at every vmult
+ *   tensor2:            deformed geometry + 2 scalars + tau/J (18 words/point;
+ *                       the data the canonical roofline counts)
+ *   tensor4:            tensor2 data + the symmetric 4th-order tangent (53 words)
+ *   tensor4_ns:         reference geometry + the full 4th-order tangent (91 words)
+ * Formulation 0 exists for tensor2 only (mf_nh_operator.h:706). */
 #define MFG_CACHING_SCALAR_REFERENTIAL 0
 #define MFG_CACHING_TENSOR2 1
 #define MFG_CACHING_TENSOR4 2

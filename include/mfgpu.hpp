@@ -148,8 +148,14 @@ class NeoHookOperator : public OperatorBase<Number> {
   };
   void initialize(const mfg_level_desc &level, const std::string &caching, const Material &mat) {
     this->clear();
-    if (caching != "tensor2") throw ExcMfg(MFG_ERR_ARGUMENT, "Unknown caching");
-    mfg_solid_params p{mat.mu, mat.nu, mat.rho, mat.alpha_1, (int32_t)mat.formulation, MFG_CACHING_TENSOR2};
+    // the "MF caching" names of the reference's parameter file (mf_nh_operator.h:346-387)
+    int32_t c;
+    if (caching == "scalar_referential") c = MFG_CACHING_SCALAR_REFERENTIAL;
+    else if (caching == "tensor2") c = MFG_CACHING_TENSOR2;
+    else if (caching == "tensor4") c = MFG_CACHING_TENSOR4;
+    else if (caching == "tensor4_ns") c = MFG_CACHING_TENSOR4_NS;
+    else throw ExcMfg(MFG_ERR_ARGUMENT, "Unknown caching");
+    mfg_solid_params p{mat.mu, mat.nu, mat.rho, mat.alpha_1, (int32_t)mat.formulation, c};
     check(mfg_solid_create(&level, &p, precision_of<Number>(), &this->op_));
   }
   template <typename UNumber>

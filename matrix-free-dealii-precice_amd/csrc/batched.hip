@@ -56,25 +56,33 @@ static int device_cus() {
   return cus;
 }
 
-template <typename Number, int WAVES>
+template <typename Number, int WAVES, int KINDS>
 static int queue_capacity() {
   static const int cap = [] {
     int nb = 0;
-    MFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batched_apply_kernel<Number, WAVES>, kBatchThreads, 0));
+    MFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batched_apply_kernel<Number, WAVES, KINDS>,
+                                                         kBatchThreads, 0));
     return std::max(1, nb) * device_cus();
   }();
   return cap;
 }
 
 // ------------------------------------------------------- apply groups ---
-// Task table over a set of instances of one precision.
+// Task tables over a set of instances of one precision: one instance table,
+// and the tasks of each instance kind present (each kind is its own launch,
+// so that a launch carries the registers of its kind's code paths only).
 struct ApplyGroup {
+  struct KindTasks {
+    int kind = 0;
+    DeviceBuffer tasks, counters;
+    std::vector<int> colour_tasks;  // tasks of colour k: [colour_tasks[k], colour_tasks[k+1])
+    int n_tasks = 0, n_slots = 0;
+  };
   int prec = MFG_FP64;
   std::vector<const OperatorData *> ops;
   std::vector<int> ids;
-  DeviceBuffer inst, tasks, counters;
-  std::vector<int> colour_tasks;  // tasks of colour k: [colour_tasks[k], colour_tasks[k+1])
-  int n_tasks = 0, n_slots = 0;
+  DeviceBuffer inst;
+  std::vector<KindTasks> kinds;
 
   ApplyGroup(std::vector<const OperatorData *> o, std::vector<int> id) : ops(std::move(o)), ids(std::move(id)) {
     MFG_REQUIRE(!ops.empty(), MFG_ERR_ARGUMENT, "empty batch");
@@ -83,6 +91,8 @@ struct ApplyGroup {
       MFG_REQUIRE(op && op->ready, MFG_ERR_STATE, "batched operator used before cache()");
       MFG_REQUIRE(op->precision == prec, MFG_ERR_ARGUMENT, "batched operators must share one precision");
       MFG_REQUIRE(op->p >= 1 && op->p <= kMaxDegree, MFG_ERR_ARGUMENT, "unsupported degree");
+      MFG_REQUIRE(op->kind == 1 || op->sp.caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
+                  "batched solid operators use the tensor2 caching");
     }
   }
 
@@ -117,12 +127,28 @@ struct ApplyGroup {
       }
       n_col = std::max(n_col, (int)op.colour_start.size() - 1);
     }
+    upload_vec(inst, hi, s);
+    kinds.clear();
+    for (int kind : {kSolidF1, kSolidF0, kHeat}) {
+      bool present = false;
+      for (int i = 0; i < n; ++i) present |= (hi[i].code >> 3) == kind;
+      if (!present) continue;
+      kinds.emplace_back();
+      build_kind(kinds.back(), kind, hi, n_col, s);
+    }
+  }
+
+  template <typename Number>
+  void build_kind(KindTasks &kt, int kind, const std::vector<BatchInst<Number>> &hi, int n_col, hipStream_t s) {
+    const int n = (int)ops.size();
+    kt.kind = kind;
     std::vector<BatchTask> tv;
     std::vector<int> last_slot(n, -1), last_count(n, 0);
-    colour_tasks.assign(1, 0);
-    n_slots = 0;
+    kt.colour_tasks.assign(1, 0);
+    int n_slots = 0;
     for (int col = 0; col < n_col; ++col) {
       for (int i = 0; i < n; ++i) {
+        if ((hi[i].code >> 3) != kind) continue;
         const OperatorData &op = *ops[i];
         const int ncol = (int)op.colour_start.size() - 1;
         const int cb = col < ncol ? op.colour_start[col] : 0, ce = col < ncol ? op.colour_start[col + 1] : 0;
@@ -150,13 +176,13 @@ struct ApplyGroup {
         last_slot[i] = slot;
         last_count[i] = groups;
       }
-      colour_tasks.push_back((int)tv.size());
+      kt.colour_tasks.push_back((int)tv.size());
     }
-    n_tasks = (int)tv.size();
-    upload_vec(inst, hi, s);
-    upload_vec(tasks, tv, s);
-    counters.allocate((size_t)(2 + n_slots) * sizeof(int));
-    MFG_HIP(hipMemsetAsync(counters.ptr, 0, counters.bytes, s));
+    kt.n_tasks = (int)tv.size();
+    kt.n_slots = n_slots;
+    upload_vec(kt.tasks, tv, s);
+    kt.counters.allocate((size_t)(2 + n_slots) * sizeof(int));
+    MFG_HIP(hipMemsetAsync(kt.counters.ptr, 0, kt.counters.bytes, s));
     MFG_HIP(hipStreamSynchronize(s));
   }
 
@@ -165,36 +191,56 @@ struct ApplyGroup {
     else build_t<float>(tu, s);
   }
 
-  template <typename Number, int WAVES>
-  void launch_t(hipStream_t s, const void *vecs, const int *active, int fix_mode, int assign, int queue,
-                int *error) const {
+  template <typename Number, int WAVES, int KINDS>
+  void launch_kind(const KindTasks &kt, hipStream_t s, const void *vecs, const int *active, int fix_mode, int assign,
+                   int queue, int *error) const {
     BatchApplyArgs<Number> a;
     a.inst = inst.as<BatchInst<Number>>();
-    a.tasks = tasks.as<BatchTask>();
+    a.tasks = kt.tasks.as<BatchTask>();
     a.vecs = (const BatchVecs<Number> *)vecs;
     a.active = active;
     a.queue = queue;
     a.fix_mode = fix_mode;
     a.assign = assign;
-    a.n_slots = n_slots;
-    a.counters = counters.as<int>();
+    a.n_slots = kt.n_slots;
+    a.counters = kt.counters.as<int>();
     a.error = error;
-    if (n_tasks == 0) return;
+    if (kt.n_tasks == 0) return;
     if (queue) {
       a.task_begin = 0;
-      a.task_end = n_tasks;
-      const int grid = std::min(n_tasks, queue_capacity<Number, WAVES>());
-      batched_apply_kernel<Number, WAVES><<<grid, kBatchThreads, 0, s>>>(a);
+      a.task_end = kt.n_tasks;
+      const int grid = std::min(kt.n_tasks, queue_capacity<Number, WAVES, KINDS>());
+      batched_apply_kernel<Number, WAVES, KINDS><<<grid, kBatchThreads, 0, s>>>(a);
       MFG_HIP(hipGetLastError());
       return;
     }
-    for (size_t col = 0; col + 1 < colour_tasks.size(); ++col) {
-      a.task_begin = colour_tasks[col];
-      a.task_end = colour_tasks[col + 1];
+    for (size_t col = 0; col + 1 < kt.colour_tasks.size(); ++col) {
+      a.task_begin = kt.colour_tasks[col];
+      a.task_end = kt.colour_tasks[col + 1];
       if (a.task_end == a.task_begin) continue;
-      batched_apply_kernel<Number, WAVES><<<a.task_end - a.task_begin, kBatchThreads, 0, s>>>(a);
+      batched_apply_kernel<Number, WAVES, KINDS><<<a.task_end - a.task_begin, kBatchThreads, 0, s>>>(a);
       MFG_HIP(hipGetLastError());
     }
+  }
+
+  template <typename Number, int WAVES>
+  void launch_t(hipStream_t s, const void *vecs, const int *active, int fix_mode, int assign, int queue,
+                int *error) const {
+    for (const KindTasks &kt : kinds) {
+      if (kt.kind == kSolidF1)
+        launch_kind<Number, WAVES, kind_bit(kSolidF1)>(kt, s, vecs, active, fix_mode, assign, queue, error);
+      else if (kt.kind == kSolidF0)
+        launch_kind<Number, WAVES, kind_bit(kSolidF0)>(kt, s, vecs, active, fix_mode, assign, queue, error);
+      else
+        launch_kind<Number, WAVES, kind_bit(kHeat)>(kt, s, vecs, active, fix_mode, assign, queue, error);
+    }
+  }
+
+  // kernel launches of one application
+  int launches(const Tuning &tu) const {
+    int l = 0;
+    for (const KindTasks &kt : kinds) l += tu.queue ? 1 : (int)kt.colour_tasks.size() - 1;
+    return l;
   }
 
   // vmult (fix_mode 1, first-touch stores) or vmult_add (fix_mode 2, adds)
@@ -524,22 +570,33 @@ struct BatchVmult {
 
 // --------------------------------------------------- batched MG and CG ---
 struct BatchSolver {
-  int n = 0, G = 0;
+  int n = 0;
   std::vector<OperatorData *> fine;
   std::vector<MGData *> mg;
   hipStream_t stream = nullptr;
   Tuning tu_fine, tu_mg;
   bool use_graph = true;
-  hipGraphExec_t graph = nullptr;
-  size_t graph_nodes = 0;
 
   std::unique_ptr<ApplyGroup> fine_group;
   std::map<std::string, std::unique_ptr<ApplyGroup>> level_groups;
-  std::vector<std::unique_ptr<DeviceBuffer>> tables;
-  std::vector<std::unique_ptr<SegTable>> segs;
-  std::vector<std::unique_ptr<TransferGroup>> transfers;
-  std::vector<std::function<void(hipStream_t)>> program;
-  int program_launches = 0;
+  // The V-cycle of a set of instances as a fixed list of launches, with the
+  // tables it reads; one program per set of instances still iterating, so
+  // converged instances launch nothing.  Captured into a hipGraph on its
+  // second use and replayed from then on.
+  struct Program {
+    std::vector<std::function<void(hipStream_t)>> steps;
+    std::vector<std::unique_ptr<DeviceBuffer>> tables;
+    std::vector<std::unique_ptr<SegTable>> segs;
+    std::vector<std::unique_ptr<TransferGroup>> transfers;
+    int launches = 0, uses = 0;
+    size_t nodes = 0;
+    hipGraphExec_t graph = nullptr;
+    ~Program() {
+      if (graph) (void)hipGraphExecDestroy(graph);
+    }
+  };
+  std::map<std::vector<int>, std::unique_ptr<Program>> programs;
+  Program *cur = nullptr;  // the program being built
 
   // outer-CG workspace (fine precision), one slice per instance
   DeviceBuffer ws_r, ws_z, ws_p, ws_h;
@@ -568,7 +625,7 @@ struct BatchSolver {
                   "the batched V-cycle needs the Chebyshev coarse solver (coarse_kind 0)");
       MFG_REQUIRE(mg[i]->levels.back().op->n_dofs == fine[i]->n_dofs, MFG_ERR_ARGUMENT,
                   "finest multigrid level and fine operator differ in size");
-      G = std::max(G, (int)mg[i]->levels.size());
+
     }
     MFG_HIP(hipDeviceSynchronize());
     MFG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -587,11 +644,9 @@ struct BatchSolver {
     build();
   }
   ~BatchSolver() {
-    if (graph) (void)hipGraphExecDestroy(graph);
-    if (stream) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipStreamDestroy(stream);
-    }
+    if (stream) (void)hipStreamSynchronize(stream);
+    programs.clear();
+    if (stream) (void)hipStreamDestroy(stream);
     if (host_active) (void)hipHostFree(host_active);
     if (host_dyn) (void)hipHostFree(host_dyn);
   }
@@ -609,17 +664,17 @@ struct BatchSolver {
   }
 
   SegTable *new_seg(std::vector<VecSeg> v, int prec, int prec_a = -1) {
-    segs.emplace_back(new SegTable());
-    SegTable *t = segs.back().get();
+    cur->segs.emplace_back(new SegTable());
+    SegTable *t = cur->segs.back().get();
     t->prec = prec;
     t->prec_a = prec_a < 0 ? prec : prec_a;
     t->build(std::move(v), stream);
     return t;
   }
   const void *new_table(const ApplyGroup &g, const std::vector<const void *> &src, const std::vector<void *> &dst) {
-    tables.emplace_back(new DeviceBuffer());
-    g.make_vecs(*tables.back(), src, dst, stream);
-    return tables.back()->ptr;
+    cur->tables.emplace_back(new DeviceBuffer());
+    g.make_vecs(*cur->tables.back(), src, dst, stream);
+    return cur->tables.back()->ptr;
   }
 
   // apply group for the instances `ins` at global level g
@@ -654,10 +709,10 @@ struct BatchSolver {
       }
       const void *vt = new_table(grp, src, dst);
       const ApplyGroup *gp = &grp;
-      program.push_back([this, gp, vt](hipStream_t s) {
+      cur->steps.push_back([this, gp, vt](hipStream_t s) {
         gp->launch(s, vt, active.as<int>(), false, tu_mg, error.as<int>());
       });
-      program_launches += tu_mg.queue ? 1 : (int)gp->colour_tasks.size() - 1;
+      cur->launches += gp->launches(tu_mg);
     }
   }
 
@@ -667,8 +722,8 @@ struct BatchSolver {
     for (auto &it : items) by_prec[it.second].push_back(it.first);
     for (auto &kv : by_prec) {
       SegTable *t = new_seg(kv.second, kv.first);
-      program.push_back([this, t](hipStream_t s) { seg_launch<OP>(*t, active.as<int>(), dyn.as<double>(), s); });
-      ++program_launches;
+      cur->steps.push_back([this, t](hipStream_t s) { seg_launch<OP>(*t, active.as<int>(), dyn.as<double>(), s); });
+      ++cur->launches;
     }
   }
 
@@ -738,41 +793,54 @@ struct BatchSolver {
       c.push_back(&mg[i]->levels[g - 1]);
       f.push_back(&mg[i]->levels[g]);
     }
-    transfers.emplace_back(new TransferGroup());
-    TransferGroup *t = transfers.back().get();
+    cur->transfers.emplace_back(new TransferGroup());
+    TransferGroup *t = cur->transfers.back().get();
     t->build(kind, tr, c, f, ins, stream);
-    program.push_back([this, t](hipStream_t s) { t->launch(s, active.as<int>()); });
-    program_launches += (int)t->colour_tasks.size() - 1;
+    cur->steps.push_back([this, t](hipStream_t s) { t->launch(s, active.as<int>()); });
+    cur->launches += (int)t->colour_tasks.size() - 1;
   }
 
-  // The V-cycle of every instance (MGData::vmult from r to z), bottom-aligned:
-  // global level g is level g of every instance that has it.
-  void build_vcycle() {
-    program.clear();
-    program_launches = 0;
-    level_groups.clear();
-    tables.clear();
-    segs.clear();
-    transfers.clear();
+  // The V-cycle (MGData::vmult from r to z) of the instances `ins`,
+  // bottom-aligned: global level g is level g of every instance that has it.
+  Program &program_for(const std::vector<int> &ins) {
+    auto it = programs.find(ins);
+    if (it != programs.end()) return *it->second;
+    std::unique_ptr<Program> pr(new Program());
+    cur = pr.get();
+    try {
+      build_vcycle(ins);
+    } catch (...) {
+      cur = nullptr;
+      throw;
+    }
+    cur = nullptr;
+    Program &ref = *pr;
+    programs[ins] = std::move(pr);
+    return ref;
+  }
+
+  void build_vcycle(const std::vector<int> &ins) {
+    int depth = 0;
+    for (int i : ins) depth = std::max(depth, (int)mg[i]->levels.size());
     auto has = [&](int g) {
       std::vector<int> v;
-      for (int i = 0; i < n; ++i)
+      for (int i : ins)
         if ((int)mg[i]->levels.size() > g) v.push_back(i);
       return v;
     };
     {  // copy_to_mg: def of the finest level = r
       std::map<int, std::vector<VecSeg>> by;
-      for (int i = 0; i < n; ++i) {
+      for (int i : ins) {
         MGLevel &T = mg[i]->levels.back();
         by[T.op->precision].push_back(make_seg(T.def.ptr, slice(ws_r, i), nullptr, nullptr, nullptr, T.op->n_dofs, i));
       }
       for (auto &kv : by) {
         SegTable *t = new_seg(kv.second, kv.first, fprec());
-        program.push_back([this, t](hipStream_t s) { seg_launch<kSegConvert>(*t, active.as<int>(), nullptr, s); });
-        ++program_launches;
+        cur->steps.push_back([this, t](hipStream_t s) { seg_launch<kSegConvert>(*t, active.as<int>(), nullptr, s); });
+        ++cur->launches;
       }
     }
-    for (int g = G - 1; g >= 1; --g) {  // down: pre-smooth, residual, restrict
+    for (int g = depth - 1; g >= 1; --g) {  // down: pre-smooth, residual, restrict
       const std::vector<int> P = has(g);
       add_chebyshev(g, P, true);
       add_apply(g, P, true);
@@ -788,27 +856,42 @@ struct BatchSolver {
       add_transfer(g, P, 1);
     }
     add_chebyshev(0, has(0), true);  // coarse: Chebyshev solver mode
-    for (int g = 1; g < G; ++g) {  // up: prolongate, post-smooth
+    for (int g = 1; g < depth; ++g) {  // up: prolongate, post-smooth
       const std::vector<int> P = has(g);
       add_transfer(g, P, 0);
       add_chebyshev(g, P, false);
     }
     {  // copy_from_mg: z = sol of the finest level
       std::map<int, std::vector<VecSeg>> by;
-      for (int i = 0; i < n; ++i) {
+      for (int i : ins) {
         MGLevel &T = mg[i]->levels.back();
         by[T.op->precision].push_back(make_seg(slice(ws_z, i), T.sol.ptr, nullptr, nullptr, nullptr, T.op->n_dofs, i));
       }
       for (auto &kv : by) {
         SegTable *t = new_seg(kv.second, fprec(), kv.first);
-        program.push_back([this, t](hipStream_t s) { seg_launch<kSegConvert>(*t, active.as<int>(), nullptr, s); });
-        ++program_launches;
+        cur->steps.push_back([this, t](hipStream_t s) { seg_launch<kSegConvert>(*t, active.as<int>(), nullptr, s); });
+        ++cur->launches;
       }
     }
-    if (graph) {
-      MFG_HIP(hipGraphExecDestroy(graph));
-      graph = nullptr;
-    }
+  }
+
+  // Programs and level groups depend on the tuning: drop them after a change.
+  void reset_programs() {
+    MFG_HIP(hipStreamSynchronize(stream));
+    programs.clear();
+    level_groups.clear();
+  }
+
+  std::vector<int> all() const {
+    std::vector<int> v(n);
+    for (int i = 0; i < n; ++i) v[i] = i;
+    return v;
+  }
+  std::vector<int> active_list() const {
+    std::vector<int> v;
+    for (int i = 0; i < n; ++i)
+      if (host_active[i]) v.push_back(i);
+    return v;
   }
 
   void build() {
@@ -836,19 +919,22 @@ struct BatchSolver {
       pr.first->prec = pr.first->prec_a = fprec();
       pr.first->build(*pr.second, stream);
     }
-    build_vcycle();
+    reset_programs();
   }
 
-  void run_vcycle() {
-    if (!use_graph) {
-      for (auto &f : program) f(stream);
+  void run_vcycle(const std::vector<int> &ins) {
+    if (ins.empty()) return;
+    Program &p = program_for(ins);
+    ++p.uses;
+    if (!use_graph || (p.uses < 2 && !p.graph)) {
+      for (auto &f : p.steps) f(stream);
       return;
     }
-    if (!graph) {
+    if (!p.graph) {
       hipGraph_t g = nullptr;
       MFG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
-        for (auto &f : program) f(stream);
+        for (auto &f : p.steps) f(stream);
       } catch (...) {
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -857,12 +943,12 @@ struct BatchSolver {
       MFG_HIP(hipStreamEndCapture(stream, &g));
       size_t nn = 0;
       (void)hipGraphGetNodes(g, nullptr, &nn);
-      graph_nodes = nn;
-      const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+      p.nodes = nn;
+      const hipError_t e = hipGraphInstantiate(&p.graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       MFG_HIP(e);
     }
-    MFG_HIP(hipGraphLaunch(graph, stream));
+    MFG_HIP(hipGraphLaunch(p.graph, stream));
   }
 
   void tune(int reps) {
@@ -888,7 +974,7 @@ struct BatchSolver {
     top.make_vecs(vt, src, dst, stream);
     top.tune(tu_mg, vt.ptr, reps, stream, error.as<int>());
     fine_group->build(tu_fine, stream);
-    build_vcycle();
+    reset_programs();
     MFG_HIP(hipStreamSynchronize(stream));
     check_error();
   }
@@ -907,7 +993,7 @@ struct BatchSolver {
     ti.build(in, stream);
     to.build(out, stream);
     seg_launch<kSegConvert>(ti, nullptr, nullptr, stream);
-    run_vcycle();
+    run_vcycle(all());
     seg_launch<kSegConvert>(to, nullptr, nullptr, stream);
     MFG_HIP(hipStreamSynchronize(stream));
     check_error();
@@ -962,7 +1048,7 @@ struct BatchSolver {
     }
     if (n_active) {
       upload_active();
-      run_vcycle();
+      run_vcycle(active_list());
       red.dot(dot_rz, act, stream);
       const std::vector<double> g0 = red.fetch(n, stream);
       for (int i = 0; i < n; ++i) gh[i] = g0[i];
@@ -1002,7 +1088,7 @@ struct BatchSolver {
         }
         if (!n_active) break;
         if (changed) upload_active();
-        run_vcycle();
+        run_vcycle(active_list());
         red.dot(dot_rz, act, stream);
         const std::vector<double> ghn = red.fetch(n, stream);
         for (int i = 0; i < n; ++i) {
@@ -1206,8 +1292,9 @@ int mfg_batch_solver_solve(mfg_batch_solver s, const void *const *b, void *const
 int mfg_batch_solver_info(mfg_batch_solver s, int32_t *vcycle_launches, int32_t *graph_nodes) {
   return guarded([&] {
     MFG_REQUIRE(s, MFG_ERR_ARGUMENT, "null solver");
-    if (vcycle_launches) *vcycle_launches = s->s->program_launches;
-    if (graph_nodes) *graph_nodes = (int32_t)s->s->graph_nodes;
+    auto &p = s->s->program_for(s->s->all());
+    if (vcycle_launches) *vcycle_launches = p.launches;
+    if (graph_nodes) *graph_nodes = (int32_t)p.nodes;
   });
 }
 

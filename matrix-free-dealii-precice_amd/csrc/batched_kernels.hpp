@@ -33,6 +33,7 @@ constexpr int kSpinLimit = 1 << 22;
 
 // instance kinds in the dispatch code (code = kind * 8 + p)
 enum : int { kSolidF1 = 0, kSolidF0 = 1, kHeat = 2 };
+constexpr int kind_bit(int kind) { return 1 << kind; }
 
 template <typename Number>
 struct BatchInst {
@@ -91,28 +92,40 @@ __device__ inline void batch_task_cells(const QOp &qop, const BatchInst<Number> 
                              s_tab + N * N, s_tile);
 }
 
-template <int P, typename Number>
+// KINDS: the instance kinds this kernel is compiled for (kind_bit mask).  A
+// launch serves the tasks of one kind, so its register allocation is that of
+// the kind's paths only, not of the union.
+template <int P, int KINDS, typename Number>
 __device__ inline void batch_task_degree(int kind, const BatchInst<Number> &in, const BatchVecs<Number> &v,
                                          const BatchTask &tk, int assign, Number *s_tab, Number *s_tile) {
-  if (kind == kSolidF1) {
-    QSolidT2F1<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1]};
-    batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
-  } else if (kind == kSolidF0) {
-    QSolidT2F0<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1], in.prm[2]};
-    batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
-  } else {
-    QHeat<Number> q{in.qd, in.nqp, in.prm[0]};
-    batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
+  if constexpr ((KINDS & kind_bit(kSolidF1)) != 0) {
+    if (kind == kSolidF1) {
+      QSolidT2F1<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1]};
+      batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
+    }
+  }
+  if constexpr ((KINDS & kind_bit(kSolidF0)) != 0) {
+    if (kind == kSolidF0) {
+      QSolidT2F0<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1], in.prm[2]};
+      batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
+    }
+  }
+  if constexpr ((KINDS & kind_bit(kHeat)) != 0) {
+    if (kind == kHeat) {
+      QHeat<Number> q{in.qd, in.nqp, in.prm[0]};
+      batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
+    }
   }
 }
 
 // WAVES: minimum waves per SIMD the register allocation must allow (2: up to
 // 256 VGPRs; 4: up to 128, which makes the fused FP32 kernel spill a little
 // but doubles its occupancy).  The tuner picks.
-template <typename Number, int WAVES>
+template <typename Number, int WAVES, int KINDS>
 __global__ void __launch_bounds__(kBatchThreads, WAVES) batched_apply_kernel(BatchApplyArgs<Number> a) {
   __shared__ Number s_tab[2 * 7 * 7];
-  __shared__ Number s_tile[9 * kBatchThreads];  // 3 NC slots of cpb*N3 <= kBatchThreads points
+  // 3 NC slots of cpb*N3 <= kBatchThreads points
+  __shared__ Number s_tile[((KINDS & ~kind_bit(kHeat)) != 0 ? 9 : 3) * kBatchThreads];
   __shared__ int s_task, s_last;
   const int t = threadIdx.x;
   for (int it = 0;; ++it) {
@@ -121,7 +134,8 @@ __global__ void __launch_bounds__(kBatchThreads, WAVES) batched_apply_kernel(Bat
       else s_task = a.task_begin + xcd_remap(blockIdx.x, gridDim.x) + it * gridDim.x;
     }
     __syncthreads();
-    const int task = s_task;
+    // uniform across the wave: the task and instance records go to scalar registers
+    const int task = __builtin_amdgcn_readfirstlane(s_task);
     if (task >= a.task_end) break;
     const BatchTask tk = a.tasks[task];
     if (a.queue && tk.wait_slot >= 0) {
@@ -151,12 +165,12 @@ __global__ void __launch_bounds__(kBatchThreads, WAVES) batched_apply_kernel(Bat
       }
       const int kind = in.code >> 3;
       switch (in.code & 7) {
-        case 1: batch_task_degree<1, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
-        case 2: batch_task_degree<2, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
-        case 3: batch_task_degree<3, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
-        case 4: batch_task_degree<4, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
-        case 5: batch_task_degree<5, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
-        case 6: batch_task_degree<6, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
+        case 1: batch_task_degree<1, KINDS, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
+        case 2: batch_task_degree<2, KINDS, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
+        case 3: batch_task_degree<3, KINDS, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
+        case 4: batch_task_degree<4, KINDS, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
+        case 5: batch_task_degree<5, KINDS, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
+        case 6: batch_task_degree<6, KINDS, Number>(kind, in, v, tk, a.assign, s_tab, s_tile); break;
         default: break;
       }
     }

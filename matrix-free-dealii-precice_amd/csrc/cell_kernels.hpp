@@ -188,6 +188,7 @@ template <typename Number>
 struct QSolidT2F1 {
   static constexpr int NC = 3;
   static constexpr int kFields = 18;
+  static constexpr bool kNodalField = false;
   const Number *qd;
   int64_t nqp;
   Number two_lambda, rho_alpha;
@@ -239,6 +240,7 @@ template <typename Number>
 struct QSolidT2F0 {
   static constexpr int NC = 3;
   static constexpr int kFields = 20;
+  static constexpr bool kNodalField = false;
   const Number *qd;
   int64_t nqp;
   Number kappa, c_1, rho_alpha;
@@ -325,6 +327,7 @@ template <typename Number>
 struct QHeat {
   static constexpr int NC = 1;
   static constexpr int kFields = 11;
+  static constexpr bool kNodalField = false;
   const Number *qd;
   int64_t nqp;
   Number dt;
@@ -347,6 +350,210 @@ struct QHeat {
 #pragma unroll
     for (int f = 0; f < 3; ++f) grad[0][f] = s * (gx[0] * d.ji[f * 3] + gx[1] * d.ji[f * 3 + 1] + gx[2] * d.ji[f * 3 + 2]);
     val[0] = d.jxw * val[0];
+  }
+};
+
+// The other formulation-1 cachings (MF caching parameter,
+// mf_nh_operator.h:848-1010).  They define the same operator as tensor2 and
+// trade memory for arithmetic differently.
+constexpr int kSymI[6] = {0, 1, 2, 0, 0, 1}, kSymJ[6] = {0, 1, 2, 1, 2, 2};
+
+// scalar_referential (mf_nh_operator.h:848-943): reference-frame geometry
+// and one scalar per point; F is rebuilt from the cached nodal displacement
+// at every application.  Fields: Jinv_ref (9), JxW_ref, mu - 2 lambda ln J.
+template <typename Number>
+struct QSolidSR {
+  static constexpr int NC = 3;
+  static constexpr int kFields = 11;
+  static constexpr bool kNodalField = true;  // reference gradient of u at the point, set by attach()
+  const Number *qd;
+  int64_t nqp;
+  const Number *u;  // nodal linearisation point, 3 per node
+  Number mu, two_lambda, rho_alpha;
+  struct Data {
+    Number ji[9], jxw, sc, ug[9];
+  };
+  __device__ Data load(int64_t q) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
+    d.jxw = stream_load(qd + 9 * nqp + q);
+    d.sc = stream_load(qd + 10 * nqp + q);
+    return d;
+  }
+  __device__ void attach(Data &d, const Number (&ug)[3][3]) const {
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ug[f] = ug[f / 3][f % 3];
+  }
+  __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
+    Number g[3][3], F[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        g[c][e] = grad[c][0] * d.ji[0 * 3 + e] + grad[c][1] * d.ji[1 * 3 + e] + grad[c][2] * d.ji[2 * 3 + e];
+        F[c][e] = d.ug[c * 3 + 0] * d.ji[0 * 3 + e] + d.ug[c * 3 + 1] * d.ji[1 * 3 + e] +
+                  d.ug[c * 3 + 2] * d.ji[2 * 3 + e] + (c == e ? Number(1) : Number(0));
+      }
+    const Number det = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                       F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+    const Number id = Number(1) / det;
+    Number Fi[3][3];
+    Fi[0][0] = (F[1][1] * F[2][2] - F[1][2] * F[2][1]) * id;
+    Fi[0][1] = (F[0][2] * F[2][1] - F[0][1] * F[2][2]) * id;
+    Fi[0][2] = (F[0][1] * F[1][2] - F[0][2] * F[1][1]) * id;
+    Fi[1][0] = (F[1][2] * F[2][0] - F[1][0] * F[2][2]) * id;
+    Fi[1][1] = (F[0][0] * F[2][2] - F[0][2] * F[2][0]) * id;
+    Fi[1][2] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * id;
+    Fi[2][0] = (F[1][0] * F[2][1] - F[1][1] * F[2][0]) * id;
+    Fi[2][1] = (F[0][1] * F[2][0] - F[0][0] * F[2][1]) * id;
+    Fi[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * id;
+    // spatial gradient of the test direction: grad_X v F^{-1}
+    Number gx[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) gx[c][e] = Fi[0][e] * g[c][0] + Fi[1][e] * g[c][1] + Fi[2][e] * g[c][2];
+    const Number trg = gx[0][0] + gx[1][1] + gx[2][2];
+    // tau = mu F F^T - (mu - 2 lambda ln J) I
+    Number tau[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        tau[a][b] = mu * (F[a][0] * F[b][0] + F[a][1] * F[b][1] + F[a][2] * F[b][2]) - (a == b ? d.sc : Number(0));
+    Number Q[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        Q[a][b] = d.sc * (gx[a][b] + gx[b][a]) + (a == b ? two_lambda * trg : Number(0)) + gx[a][0] * tau[0][b] +
+                  gx[a][1] * tau[1][b] + gx[a][2] * tau[2][b];
+    // first Piola form: (jc + gx tau) F^{-T}, then to reference gradients
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Number Pm[3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Pm[b] = Q[c][0] * Fi[b][0] + Q[c][1] * Fi[b][1] + Q[c][2] * Fi[b][2];
+#pragma unroll
+      for (int f = 0; f < 3; ++f) grad[c][f] = d.jxw * (Pm[0] * d.ji[f * 3 + 0] + Pm[1] * d.ji[f * 3 + 1] + Pm[2] * d.ji[f * 3 + 2]);
+      val[c] = d.jxw * rho_alpha * val[c];
+    }
+  }
+};
+
+// tensor4 (mf_nh_operator.h:976-994): deformed-frame geometry, 1/J, tau/J
+// and the symmetric fourth-order tangent C = (2 s_c/J) S + (2 lambda/J) I x I
+// as a 6 x 6 table over the symmetric index pairs (xx,yy,zz,xy,xz,yz).
+// Fields: Jinv_cur (9), JxW_cur, 1/J, tau/J (6), C (36); the C entries are
+// read while contracting.
+template <typename Number>
+struct QSolidT4 {
+  static constexpr int NC = 3;
+  static constexpr int kFields = 53;
+  static constexpr bool kNodalField = false;
+  const Number *qd;
+  int64_t nqp;
+  Number rho_alpha;
+  struct Data {
+    Number ji[9], jxw, c1, T[6];
+    int64_t q;
+  };
+  __device__ Data load(int64_t q) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
+    d.jxw = stream_load(qd + 9 * nqp + q);
+    d.c1 = stream_load(qd + 10 * nqp + q);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) d.T[f] = stream_load(qd + (11 + f) * nqp + q);
+    d.q = q;
+    return d;
+  }
+  __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
+    Number g[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) g[c][e] = grad[c][0] * d.ji[0 * 3 + e] + grad[c][1] * d.ji[1 * 3 + e] + grad[c][2] * d.ji[2 * 3 + e];
+    // C : sym(g), summing both orders of the off-diagonal pairs
+    Number sym2[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+      sym2[t] = (t < 3 ? Number(0.5) : Number(1)) * (g[kSymI[t]][kSymJ[t]] + g[kSymJ[t]][kSymI[t]]);
+    Number Cs[6];
+#pragma unroll
+    for (int s6 = 0; s6 < 6; ++s6) {
+      Number acc = 0;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) acc += stream_load(qd + (17 + 6 * s6 + t) * nqp + d.q) * sym2[t];
+      Cs[s6] = acc;
+    }
+    const Number Cm[3][3] = {{Cs[0], Cs[3], Cs[4]}, {Cs[3], Cs[1], Cs[5]}, {Cs[4], Cs[5], Cs[2]}};
+    const Number Tm[3][3] = {{d.T[0], d.T[3], d.T[4]}, {d.T[3], d.T[1], d.T[5]}, {d.T[4], d.T[5], d.T[2]}};
+    Number Pm[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Pm[a][b] = Cm[a][b] + g[a][0] * Tm[0][b] + g[a][1] * Tm[1][b] + g[a][2] * Tm[2][b];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        grad[c][f] = d.jxw * (Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2]);
+      val[c] = d.jxw * rho_alpha * d.c1 * val[c];
+    }
+  }
+};
+
+// tensor4_ns (mf_nh_operator.h:996-1010): reference-frame geometry and the
+// full (non-symmetric) fourth-order tangent A (81 entries), contracted as
+// double_contract<2,0,3,1>(A, Grad v).  Fields: Jinv_ref (9), JxW_ref, A (81).
+template <typename Number>
+struct QSolidT4NS {
+  static constexpr int NC = 3;
+  static constexpr int kFields = 91;
+  static constexpr bool kNodalField = false;
+  const Number *qd;
+  int64_t nqp;
+  Number rho_alpha;
+  struct Data {
+    Number ji[9], jxw;
+    int64_t q;
+  };
+  __device__ Data load(int64_t q) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
+    d.jxw = stream_load(qd + 9 * nqp + q);
+    d.q = q;
+    return d;
+  }
+  __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
+    Number g[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) g[c][e] = grad[c][0] * d.ji[0 * 3 + e] + grad[c][1] * d.ji[1 * 3 + e] + grad[c][2] * d.ji[2 * 3 + e];
+    Number Pm[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        Number acc = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) acc += stream_load(qd + (10 + ((a * 3 + b) * 3 + k) * 3 + l) * nqp + d.q) * g[k][l];
+        Pm[a][b] = acc;
+      }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        grad[c][f] = d.jxw * (Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2]);
+      val[c] = d.jxw * rho_alpha * val[c];
+    }
   }
 };
 
@@ -413,8 +620,17 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
     const bool skip = !active || first || ((entry >> (kConstraintShift + c)) & 1u);
     dold[c] = skip ? Number(0) : dst[node * NC + c];
   }
-  const typename QOp::Data qd = qop.load(qidx);
+  typename QOp::Data qd = qop.load(qidx);
   __syncthreads();
+  if constexpr (QOp::kNodalField) {
+    // reference gradient of the cached nodal field (read plain: constrained
+    // entries as stored), mf_nh_operator.h:873-877
+    Number w[3], wg[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[c] = active ? qop.u[node * 3 + c] : Number(0);
+    evaluate_g<N, 3, Number>(T, s_shape, s_colloc, w, wg);
+    qop.attach(qd, wg);
+  }
 
   Number val[NC], grad[NC][3];
   evaluate_vg<N, NC, Number>(T, s_shape, s_colloc, u, val, grad);

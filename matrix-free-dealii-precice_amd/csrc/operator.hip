@@ -57,11 +57,21 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
     }
   };
   if (op.kind == 0) {
-    if (op.sp.formulation == 1) {
-      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+    const Number ra = Number(op.rho_alpha);
+    if (op.sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {
+      QSolidSR<Number> q{op.qd.as<Number>(), op.nqp, op.u_nodal.as<Number>(), Number(op.sp.mu), Number(2.0 * op.lambda), ra};
+      launch_simple(apply_cells<P, CPB, Number, QSolidSR<Number>>, q);
+    } else if (op.sp.caching == MFG_CACHING_TENSOR4) {
+      QSolidT4<Number> q{op.qd.as<Number>(), op.nqp, ra};
+      launch_simple(apply_cells<P, CPB, Number, QSolidT4<Number>>, q);
+    } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
+      QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
+      launch_simple(apply_cells<P, CPB, Number, QSolidT4NS<Number>>, q);
+    } else if (op.sp.formulation == 1) {
+      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), ra};
       launch_simple(apply_cells<P, CPB, Number, QSolidT2F1<Number>>, q);
     } else {
-      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
+      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), ra};
       launch_simple(apply_cells<P, CPB, Number, QSolidT2F0<Number>>, q);
     }
   } else {
@@ -88,6 +98,7 @@ static void launch_cache(OperatorData &op, const void *u, int u_fp32) {
   a.mu = Number(op.sp.mu);
   a.lambda = Number(op.lambda);
   a.formulation = op.sp.formulation;
+  a.caching = op.sp.caching;
   a.error = op.flag.as<int>();
   const int grid = blocks_for(op.n_cells, CPB);
   cache_solid<P, CPB, Number><<<grid, block_threads<P, CPB>(), 0, op.stream>>>(a);
@@ -99,6 +110,8 @@ static void launch_diagonal(OperatorData &op) {
   DiagArgs<Number> a;
   a.cell_nodes = op.cell_nodes.as<uint32_t>();
   a.sq = op.tab_sq.as<Number>();
+  a.shape = op.tab_shape.as<Number>();
+  a.colloc = op.tab_colloc.as<Number>();
   a.diag = op.diag.as<Number>();
   constexpr int NT = ((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64;
   for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
@@ -107,7 +120,17 @@ static void launch_diagonal(OperatorData &op) {
     const int grid = a.cell_end - a.cell_begin;
     if (grid == 0) continue;
     if (op.kind == 0) {
-      if (op.sp.formulation == 1) {
+      const Number ra = Number(op.rho_alpha);
+      if (op.sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {
+        QSolidSR<Number> q{op.qd.as<Number>(), op.nqp, op.u_nodal.as<Number>(), Number(op.sp.mu), Number(2.0 * op.lambda), ra};
+        diagonal_cells<P, Number, QSolidSR<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      } else if (op.sp.caching == MFG_CACHING_TENSOR4) {
+        QSolidT4<Number> q{op.qd.as<Number>(), op.nqp, ra};
+        diagonal_cells<P, Number, QSolidT4<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
+        QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
+        diagonal_cells<P, Number, QSolidT4NS<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      } else if (op.sp.formulation == 1) {
         QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
         diagonal_cells<P, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
       } else {
@@ -233,6 +256,19 @@ void OperatorData::cache(const void *u, int u_precision) {
   MFG_REQUIRE(kind == 0, MFG_ERR_ARGUMENT, "cache() applies to the solid operator");
   MFG_REQUIRE(u, MFG_ERR_ARGUMENT, "null displacement");
   fill<int>(flag.as<int>(), 1, 0, stream);
+  if (sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {  // keeps the displacement itself (mf_nh_operator.h:442-455)
+    const int threads = 256, blocks = std::min(blocks_for(n_dofs, threads), 65536);
+    if (u_nodal.bytes != (size_t)n_dofs * elem_size()) u_nodal.allocate((size_t)n_dofs * elem_size());
+    if (precision == MFG_FP64 && u_precision == MFG_FP64)
+      convert_kernel<double, double><<<blocks, threads, 0, stream>>>(u_nodal.as<double>(), (const double *)u, n_dofs);
+    else if (precision == MFG_FP64)
+      convert_kernel<double, float><<<blocks, threads, 0, stream>>>(u_nodal.as<double>(), (const float *)u, n_dofs);
+    else if (u_precision == MFG_FP64)
+      convert_kernel<float, double><<<blocks, threads, 0, stream>>>(u_nodal.as<float>(), (const double *)u, n_dofs);
+    else
+      convert_kernel<float, float><<<blocks, threads, 0, stream>>>(u_nodal.as<float>(), (const float *)u, n_dofs);
+    MFG_HIP(hipGetLastError());
+  }
   if (precision == MFG_FP64) cache_t<double>(*this, u, u_precision == MFG_FP32);
   else cache_t<float>(*this, u, u_precision == MFG_FP32);
   int h = 0;
@@ -416,13 +452,18 @@ void OperatorData::setup_solid() {
   c_1 = mu / 2.0;
   lambda = (2.0 * mu * nu) / (1.0 - 2.0 * nu);
   rho_alpha = sp.rho * sp.alpha_1;
-  n_fields = sp.formulation == 1 ? 18 : 20;
+  switch (sp.caching) {
+    case MFG_CACHING_SCALAR_REFERENTIAL: n_fields = QSolidSR<double>::kFields; break;
+    case MFG_CACHING_TENSOR4: n_fields = QSolidT4<double>::kFields; break;
+    case MFG_CACHING_TENSOR4_NS: n_fields = QSolidT4NS<double>::kFields; break;
+    default: n_fields = sp.formulation == 1 ? QSolidT2F1<double>::kFields : QSolidT2F0<double>::kFields;
+  }
   const size_t es = precision == MFG_FP64 ? 8 : 4;
   qd.allocate((size_t)n_fields * nqp * es);
 }
 
 size_t OperatorData::memory_consumption() const {
-  return qd.bytes + cell_nodes.bytes + constrained.bytes;
+  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes;
 }
 
 }  // namespace mfg
@@ -437,8 +478,11 @@ int mfg_solid_create(const mfg_level_desc *level, const mfg_solid_params *params
     MFG_REQUIRE(precision == MFG_FP64 || precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
     MFG_REQUIRE(params->formulation == 0 || params->formulation == 1, MFG_ERR_ARGUMENT,
                 "Unknown material formulation");
-    MFG_REQUIRE(params->caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
-                "the device operator implements the tensor2 caching");
+    MFG_REQUIRE(params->caching >= MFG_CACHING_SCALAR_REFERENTIAL && params->caching <= MFG_CACHING_TENSOR4_NS,
+                MFG_ERR_ARGUMENT, "Unknown caching strategy");
+    // mf_nh_operator.h:706: formulation 0 is implemented for tensor2 only
+    MFG_REQUIRE(params->formulation == 1 || params->caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
+                "formulation 0 needs the tensor2 caching");
     MFG_REQUIRE(params->mu > 0 && params->nu > -1 && params->nu < 0.5, MFG_ERR_ARGUMENT, "bad material");
     auto op = new mfg_operator_t();
     try {

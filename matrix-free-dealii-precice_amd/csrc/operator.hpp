@@ -20,6 +20,7 @@ struct OperatorData {
   DeviceBuffer tab_shape, tab_colloc, tab_sq, tab_points, tab_weights;
   DeviceBuffer qd;           // quadrature cache, field-major SoA
   DeviceBuffer diag, inv_diag;
+  DeviceBuffer u_nodal;      // scalar_referential caching: the linearisation point
   DeviceBuffer flag;         // int error flag
   std::vector<int32_t> host_constrained;
   std::vector<int32_t> host_cell_nodes;  // input cell order (used to build transfers)

@@ -20,7 +20,8 @@ struct CacheArgs {
   int64_t nqp;
   Number mu, lambda;
   int formulation;
-  int *error;  // set to 1 if det F <= 0 anywhere
+  int caching;  // MFG_CACHING_*
+  int *error;   // set to 1 if det F <= 0 anywhere
 };
 
 template <typename Number>
@@ -133,9 +134,39 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
     atomicOr(a.error, 1);
     return;
   }
-  inv3(Jc, Jci);
-  const Number jxw = (Number)(a.weights[T.i] * a.weights[T.j] * a.weights[T.k]) * fabs(det3(Jc));
   const int64_t n = a.nqp;
+  const double wq = a.weights[T.i] * a.weights[T.j] * a.weights[T.k];
+  const Number scalar = a.mu - Number(2) * a.lambda * log(J);
+  if (a.caching == MFG_CACHING_SCALAR_REFERENTIAL || a.caching == MFG_CACHING_TENSOR4_NS) {
+    // reference-frame geometry (MappingQ1 of the reference configuration)
+    const double detr = Jr[0][0] * (Jr[1][1] * Jr[2][2] - Jr[1][2] * Jr[2][1]) -
+                        Jr[0][1] * (Jr[1][0] * Jr[2][2] - Jr[1][2] * Jr[2][0]) +
+                        Jr[0][2] * (Jr[1][0] * Jr[2][1] - Jr[1][1] * Jr[2][0]);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) a.qd[f * n + qidx] = (Number)Jri[f / 3][f % 3];
+    a.qd[9 * n + qidx] = (Number)(wq * fabs(detr));
+    if (a.caching == MFG_CACHING_SCALAR_REFERENTIAL) {  // mf_nh_operator.h:442-455
+      a.qd[10 * n + qidx] = scalar;
+      return;
+    }
+    // tensor4_ns (mf_nh_operator.h:495-516): 2 lambda F^-T (x) F^-T + s_c F^-T (x)_il F^-1 + mu I (x)_ik I
+    Number Fi[3][3];
+    inv3(F, Fi);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l)
+            a.qd[(10 + ((i * 3 + j) * 3 + k) * 3 + l) * n + qidx] =
+                Number(2) * a.lambda * Fi[j][i] * Fi[l][k] + scalar * Fi[l][i] * Fi[j][k] +
+                ((i == k && j == l) ? a.mu : Number(0));
+    return;
+  }
+  inv3(Jc, Jci);
+  const Number jxw = (Number)wq * fabs(det3(Jc));
 #pragma unroll
   for (int f = 0; f < 9; ++f) a.qd[f * n + qidx] = Jci[f / 3][f % 3];
   a.qd[9 * n + qidx] = jxw;
@@ -146,21 +177,38 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
     for (int f = 0; f < 9; ++f) a.qd[(11 + f) * n + qidx] = F[f / 3][f % 3];
     return;
   }
-  // mf_nh_operator.h:456-477
-  const Number scalar = a.mu - Number(2) * a.lambda * log(J);
   const Number invJ = Number(1) / J;
-  a.qd[10 * n + qidx] = scalar * Number(2) / J;
-  a.qd[11 * n + qidx] = invJ;
   Number b[3][3];
 #pragma unroll
   for (int x = 0; x < 3; ++x)
 #pragma unroll
     for (int y = 0; y < 3; ++y) b[x][y] = F[x][0] * F[y][0] + F[x][1] * F[y][1] + F[x][2] * F[y][2];
-  const int ii[6] = {0, 1, 2, 0, 0, 1}, jj[6] = {0, 1, 2, 1, 2, 2};
+  if (a.caching == MFG_CACHING_TENSOR4) {  // mf_nh_operator.h:478-494
+    a.qd[10 * n + qidx] = invJ;
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
-    Number tau = a.mu * b[ii[s]][jj[s]] - (ii[s] == jj[s] ? scalar : Number(0));
-    a.qd[(12 + s) * n + qidx] = tau / J;
+    for (int s6 = 0; s6 < 6; ++s6) {
+      const Number tau = a.mu * b[kSymI[s6]][kSymJ[s6]] - (kSymI[s6] == kSymJ[s6] ? scalar : Number(0));
+      a.qd[(11 + s6) * n + qidx] = tau / J;
+    }
+    // C_ijkl = (2 s_c / J) (d_ik d_jl + d_il d_jk)/2 + (2 lambda / J) d_ij d_kl over symmetric pairs
+#pragma unroll
+    for (int s6 = 0; s6 < 6; ++s6)
+#pragma unroll
+      for (int t6 = 0; t6 < 6; ++t6) {
+        const int i = kSymI[s6], j = kSymJ[s6], k = kSymI[t6], l = kSymJ[t6];
+        const Number S = Number(0.5) * (Number((i == k && j == l) ? 1 : 0) + Number((i == l && j == k) ? 1 : 0));
+        const Number IxI = (i == j && k == l) ? Number(1) : Number(0);
+        a.qd[(17 + 6 * s6 + t6) * n + qidx] = (scalar * Number(2) / J) * S + (a.lambda * Number(2) / J) * IxI;
+      }
+    return;
+  }
+  // tensor2, mf_nh_operator.h:456-477
+  a.qd[10 * n + qidx] = scalar * Number(2) / J;
+  a.qd[11 * n + qidx] = invJ;
+#pragma unroll
+  for (int s6 = 0; s6 < 6; ++s6) {
+    const Number tau = a.mu * b[kSymI[s6]][kSymJ[s6]] - (kSymI[s6] == kSymJ[s6] ? scalar : Number(0));
+    a.qd[(12 + s6) * n + qidx] = tau / J;
   }
 }
 
@@ -176,6 +224,7 @@ struct DiagArgs {
   const uint32_t *cell_nodes;
   int cell_begin, cell_end;  // one colour: cells share no node
   const Number *sq;  // 3 x N x N: SS, SD, DD
+  const Number *shape, *colloc;  // for ops that read a nodal field
   Number *diag;
 };
 
@@ -200,7 +249,20 @@ __global__ void __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) 
   const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
   Number coef[NTERM];
   {
-    const typename QOp::Data qd = qop.load(qidx);
+    typename QOp::Data qd = qop.load(qidx);
+    if constexpr (QOp::kNodalField) {
+      __shared__ Number s_shape[N * N], s_colloc[N * N];
+      if (t < N * N) {
+        s_shape[t] = a.shape[t];
+        s_colloc[t] = a.colloc[t];
+      }
+      const int64_t node = active ? (int64_t)(a.cell_nodes[qidx] & kNodeMaskDev) : 0;
+      Number w[3], wg[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) w[c] = active ? qop.u[node * 3 + c] : Number(0);
+      evaluate_g<N, 3, Number>(T, s_shape, s_colloc, w, wg);
+      qop.attach(qd, wg);
+    }
     Number v[NC], g[NC][3];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {

@@ -173,6 +173,43 @@ def test_colouring_and_bit_reproducibility(cuda):
     assert mfgpu.NeoHookOperator(one, O.MU, O.NU, O.RHO, O.ALPHA1).n_colours() == 1
 
 
+ORACLE_CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
+
+
+@pytest.mark.parametrize("caching", ["scalar_referential", "tensor4", "tensor4_ns"])
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_solid_other_cachings(cuda, caching, p):
+    """The other MF cachings against the oracle's implementation of the same
+    caching, against the device tensor2 operator (all four define one
+    operator), diagonal included; FP32 to the FP32 bar."""
+    print(f"caching {caching} p={p}", flush=True)
+    lo, hi, n = [0, 0, 0], [1.0, 0.6, 0.4], [3, 2, 2]
+    gm = mfgpu.BoxMesh(lo, hi, n, p, 3, 1, 7)
+    om = O.Mesh(lo, hi, n, p, 3, 1, 7)
+    u = om.u0(amp=5e-2, noise=1e-3)
+    oo = O.Solid(om, caching=ORACLE_CACHING[caching])
+    oo.cache(u)
+    x = O.fill_uniform(3, om.n_dofs)
+    ref = oo.vmult(x)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, caching=caching)
+    go.cache(dev(u))
+    y = torch.empty(om.n_dofs, dtype=torch.float64, device="cuda")
+    go.vmult(y, dev(x))
+    assert rel(y, ref) < TOL64
+    t2 = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, caching="tensor2")
+    t2.cache(dev(u))
+    y2 = torch.empty_like(y)
+    t2.vmult(y2, dev(x))
+    assert rel(y, y2.cpu().numpy()) < TOL64
+    go.compute_diagonal()
+    assert rel(go.diagonal(), oo.diagonal()) < TOL64
+    g32 = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, caching=caching, precision=mfgpu.FP32)
+    g32.cache(dev(u, torch.float32))
+    y32 = torch.empty(om.n_dofs, dtype=torch.float32, device="cuda")
+    g32.vmult(y32, dev(x, torch.float32))
+    assert rel(y32.double(), ref) < TOL32
+
+
 def test_error_paths(cuda):
     gm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
     go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
@@ -188,5 +225,5 @@ def test_error_paths(cuda):
     with pytest.raises(mfgpu.MfgError) as e:
         go.cache(dev(u))
     assert e.value.code == 3 and "det_F" in str(e.value)
-    with pytest.raises(mfgpu.MfgError):
-        mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, caching="tensor4")
+    with pytest.raises(mfgpu.MfgError):  # formulation 0 exists for tensor2 only (mf_nh_operator.h:706)
+        mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, formulation=0, caching="tensor4")
