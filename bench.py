@@ -119,10 +119,10 @@ def mgcg_bench(mfgpu, torch, cells, p, level_precision, x_true_seed=2):
             "err_vs_x_true": err}, x
 
 
-def batched_instances(mfgpu, n_inst=48, seed_base=3, coarse_eig_iterations=100):
+def batched_instances(mfgpu, n_inst=48, seed_base=3, coarse_eig_iterations=100, n_range=(4, 24)):
     """Instance k (seed 3+k): k < n/2 heat (M + dt c K, dt 0.1, c ~ U[0.5, 2], u = 0 on the boundary),
     else solid (neo-Hookean tensor2, u = 0 on x = 0); p ~ U{1..6}, n_q = p+1; unit cube with
-    n_x, n_y, n_z ~ U{4..24}.  FP64 fine operator, FP32 MG (levels halve while all counts are even;
+    n_x, n_y, n_z ~ U{n_range}.  FP64 fine operator, FP32 MG (levels halve while all counts are even;
     Chebyshev degree 5 on levels > 0, Chebyshev solver on level 0)."""
     import numpy as np
     out = []
@@ -130,21 +130,23 @@ def batched_instances(mfgpu, n_inst=48, seed_base=3, coarse_eig_iterations=100):
         r = np.random.default_rng(seed_base + k)
         kind = "heat" if k < n_inst // 2 else "solid"
         p = int(r.integers(1, 7))
-        n = [int(x) for x in r.integers(4, 25, size=3)]
+        n = [int(x) for x in r.integers(n_range[0], n_range[1] + 1, size=3)]
         c = float(r.uniform(0.5, 2.0))
         out.append(mfgpu.Instance(kind, n, p, fine_precision=mfgpu.FP64, level_precision=mfgpu.FP32, dt=0.1,
                                   conductivity=c, coarse_eig_iterations=coarse_eig_iterations, seed=seed_base + k))
     return out
 
 
-def batched_bench(mfgpu, torch, reps=20):
+def batched_bench(mfgpu, torch, reps=20, n_range=(4, 24)):
     """Batched vmult (one launch for all 48 FP64 fine operators) and batched
     MG-CG (FP64 outer CG, FP32 V-cycle as a replayed hipGraph), each against
-    the same work done instance by instance on the same GPU."""
+    the same work done instance by instance on the same GPU.  The batched
+    solve is timed in steady state (third solve: every V-cycle program of the
+    solve already captured)."""
     import numpy as np
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    inst = batched_instances(mfgpu)
+    inst = batched_instances(mfgpu, n_range=n_range)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     log(f"batched: {len(inst)} instances set up in {t_setup:.1f} s")
@@ -183,10 +185,11 @@ def batched_bench(mfgpu, torch, reps=20):
         xts.append(xt)
     solver = mfgpu.BatchSolver(fine, [i.mg for i in inst])
     fine_t, level_t = solver.tune(repetitions=3)
-    xs = [torch.zeros_like(b) for b in bs]
     log("batched: solver tuned")
-    solver.solve(bs, xs, tol_rel=1e-8)  # first call captures the graph
-    log("batched: first solve done")
+    for _ in range(2):  # programs run eagerly on first use and are captured on the second
+        xs = [torch.zeros_like(b) for b in bs]
+        solver.solve(bs, xs, tol_rel=1e-8)
+    log("batched: warm-up solves done")
     xs = [torch.zeros_like(b) for b in bs]
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -208,7 +211,7 @@ def batched_bench(mfgpu, torch, reps=20):
     its = [i.iterations for i in infos]
     return {
         "workload": "48 instances (seed 3+k): 24 heat (M + 0.1 c K, c ~ U[0.5,2]) + 24 neo-Hookean solids, "
-                    "p ~ U{1..6}, unit cube n ~ U{4..24}^3, FP64 fine operators, FP32 MG",
+                    f"p ~ U{{1..6}}, unit cube n ~ U{{{n_range[0]}..{n_range[1]}}}^3, FP64 fine operators, FP32 MG",
         "n_instances": len(inst), "n_dofs_total": int(n_dofs), "setup_s": round(t_setup, 2),
         "levels": [len(i.meshes) for i in inst],
         "vmult": {"batched_ms": round(t_batch_vmult * 1e3, 4), "sequential_ms": round(t_seq_vmult * 1e3, 4),
@@ -270,6 +273,37 @@ def serving_bench(mfgpu, n_sessions=32, n_requests=64, rate=40.0, seed=100, max_
     }
 
 
+def cachings_bench(mfgpu, torch, cells, p, reps=20):
+    """The four MF cachings (same operator) on the beam: FP64 vmult time and
+    the words each stores per quadrature point (the memory/arithmetic trade)."""
+    dev = torch.device("cuda:0")
+    mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], cells, p, 3, 1, 7)
+    u0 = torch.from_numpy(mesh.u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
+    src = mfgpu.fill_uniform(1, torch.empty(mesh.n_dofs, dtype=torch.float64, device=dev))
+    out, ref = {}, None
+    for name in ("scalar_referential", "tensor2", "tensor4", "tensor4_ns"):
+        op = mfgpu.NeoHookOperator(mesh, MU, NU, RHO, ALPHA1, caching=name)
+        op.cache(u0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            op.vmult(dst, src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.vmult(dst, src)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        if ref is None:
+            ref = dst.clone()
+        out[name] = {"ms": round(ms, 4), "gdofs": round(mesh.n_dofs / (ms * 1e-3) / 1e9, 3),
+                     "cache_words_per_qp": op.n_fields(),
+                     "rel_diff_vs_scalar_referential": float(torch.linalg.norm(dst - ref) / torch.linalg.norm(ref))}
+        log(f"caching {name}: {ms:.3f} ms")
+        del op
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +314,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--only-batched", action="store_true", help="run the batched-instances leg alone")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-cachings", action="store_true", help="skip the MF-caching comparison leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: 2 vmults, then 2 copies of a 1 GiB buffer (known bytes), no timing")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
@@ -298,7 +333,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     dev = torch.device("cuda:0")
     if args.only_batched:
-        print(json.dumps({"batched": batched_bench(mfgpu, torch)}), flush=True)
+        print(json.dumps({"batched": batched_bench(mfgpu, torch),
+                          "batched_small": batched_bench(mfgpu, torch, n_range=(2, 6))}), flush=True)
         return
     if args.only_serving:
         print(json.dumps({"serving": serving_bench(mfgpu)}), flush=True)
@@ -388,9 +424,14 @@ def main():
         out["mgcg"] = {"workload": "MG-CG to ||r||/||b|| <= 1e-8, rhs = A x_true (seed 2), Chebyshev degree 5, "
                                    "coarse FP64 CG to 1e-3", "fp64": r64, "mixed_fp32_vcycle": r32,
                        "speedup_mixed_vs_fp64": round(r64["solve_s"] / r32["solve_s"], 3)}
+    if not args.no_cachings:
+        torch.cuda.empty_cache()
+        out["cachings"] = cachings_bench(mfgpu, torch, args.cells, p)
     if not args.no_batched:
         torch.cuda.empty_cache()
         out["batched"] = batched_bench(mfgpu, torch)
+        torch.cuda.empty_cache()
+        out["batched_small"] = batched_bench(mfgpu, torch, n_range=(2, 6))
     if not args.no_serving:
         torch.cuda.empty_cache()
         out["serving"] = serving_bench(mfgpu)

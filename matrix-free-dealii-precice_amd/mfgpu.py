@@ -269,6 +269,12 @@ class _Operator:
         check(lib().mfg_operator_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def n_fields(self):
+        """Words stored per quadrature point by the operator's cache."""
+        nf = ctypes.c_int32(0)
+        check(lib().mfg_operator_qp_data(self.h, None, ctypes.byref(nf)))
+        return nf.value
+
     def qp_data(self):
         import torch
         nf = ctypes.c_int32(0)
