@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libmfgpu.so")
 
 FP64, FP32 = 0, 1
 CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
+CELLS_COLOURS, CELLS_BUNDLES, CELLS_PENCILS = 0, 1, 2
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
 _c_dp = ctypes.POINTER(ctypes.c_double)
@@ -43,6 +44,8 @@ _LIB = None
 
 # name -> (restype, argtypes); the exported surface of include/mfgpu.h
 SIGNATURES = {
+    "mfg_operator_set_cell_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mfg_operator_cell_schedule": (ctypes.c_int32, [ctypes.c_void_p]),
     "mfg_last_error": (ctypes.c_char_p, []),
     "mfg_version": (ctypes.c_char_p, []),
     "mfg_box_mesh_create": (ctypes.c_int, [_c_dp, _c_dp, _c_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -268,6 +271,13 @@ class _Operator:
         ms, n = ctypes.c_double(0), ctypes.c_int64(0)
         check(lib().mfg_operator_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def set_cell_schedule(self, schedule):
+        """CELLS_COLOURS or CELLS_BUNDLES (structured boxes; falls back to colours elsewhere)."""
+        check(lib().mfg_operator_set_cell_schedule(self.h, schedule))
+
+    def cell_schedule(self):
+        return lib().mfg_operator_cell_schedule(self.h)
 
     def n_fields(self):
         """Words stored per quadrature point by the operator's cache."""
@@ -710,7 +720,7 @@ class Instance:
     def __init__(self, kind, n_cells, degree, fine_precision=FP64, level_precision=FP32, lo=(0.0, 0.0, 0.0),
                  hi=(1.0, 1.0, 1.0), dt=0.1, conductivity=1.0, formulation=1, cheb_degree=5, smoothing_range=None,
                  eig_cg_iterations=None, coarse_eig_iterations=0, amp=1e-3, noise=1e-5, seed=0,
-                 material=(384615.38461538462, 0.3, 1000.0, 4e6)):
+                 material=(384615.38461538462, 0.3, 1000.0, 4e6), cell_schedule=None):
         import numpy as np
         import torch
         self.kind, self.degree = kind, degree
@@ -730,6 +740,9 @@ class Instance:
                 return LaplaceOperator(m, dt, coef, precision=pr)
         self.levels = [make(m, level_precision) for m in self.meshes]
         self.fine = make(fine_mesh, fine_precision)
+        if cell_schedule is not None:
+            for op in self.levels + [self.fine]:
+                op.set_cell_schedule(cell_schedule)
         self.transfers = [Transfer(self.levels[l - 1], self.levels[l], box_children(self.meshes[l - 1], self.meshes[l]))
                           for l in range(1, len(self.meshes))]
         if solid:

@@ -2,8 +2,9 @@
 and solid operators (degrees 1..6, FP64 and FP32), the batched V-cycle and the
 lockstep CG.  Every instance performs the operation sequence of its own
 per-instance call, so the bar is equality with the per-instance results
-(bitwise; the per-instance paths are themselves pinned to the oracle by the
-operator and multigrid tests)."""
+(bitwise, with the per-instance operators on the colour cell schedule that
+the batched kernels use; the per-instance paths are themselves pinned to the
+oracle by the operator and multigrid tests)."""
 import numpy as np
 import pytest
 import torch
@@ -27,6 +28,7 @@ def make_ops(specs, precision):
             m = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], n, p, 1, 63, 1)
             coef = np.full(m.n_cells * (p + 1) ** 3, 0.5 + 0.1 * k)
             op = mfgpu.LaplaceOperator(m, 0.1, coef, precision=precision)
+        op.set_cell_schedule(mfgpu.CELLS_COLOURS)  # the batched kernels' cell order: bitwise comparable
         ops.append(op)
         meshes.append(m)
     return ops, meshes
@@ -98,7 +100,7 @@ def instances(specs, level_precision=mfgpu.FP32):
     out = []
     for k, (kind, n, p) in enumerate(specs):
         out.append(mfgpu.Instance(kind, n, p, level_precision=level_precision, conductivity=0.5 + 0.25 * k,
-                                  coarse_eig_iterations=40, seed=k))
+                                  coarse_eig_iterations=40, seed=k, cell_schedule=mfgpu.CELLS_COLOURS))
     return out
 
 
