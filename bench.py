@@ -24,11 +24,13 @@ ALPHA1 = 1.0 / (BETA * DT * DT)
 HBM_PEAK_GBS = 8000.0
 
 BEAM = dict(lo=[0.0, 0.0, 0.0], hi=[4.0, 1.0, 1.0], cells=[128, 32, 32], degree=4)
+# node numbering of every bench mesh (the vector layout in HBM); "cellwise" gives each cell a contiguous block
+NUMBERING = "cellwise"
 
 # HBM bytes per vmult of the current cell kernel from two rocprofv3 --pmc passes (FETCH_SIZE x 2.00 and
 # WRITE_SIZE x 1.00, factors from a copy of known size); PMC counters cannot be read from inside this
 # process, so the figure is re-measured whenever the kernel changes.
-MEASURED_TRAFFIC = {((128, 32, 32), 4): 4017419059}
+MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise"): 3682208154, ((128, 32, 32), 4, "lexicographic"): 4017419059}
 
 
 def log(msg):
@@ -68,7 +70,7 @@ def cpu_baseline(seconds_budget=20.0):
                       f"as the 128x32x32 beam), median of {len(times)} vmults, {threads} OpenMP threads"}
 
 
-def mgcg_bench(mfgpu, torch, cells, p, level_precision, x_true_seed=2):
+def mgcg_bench(mfgpu, torch, cells, p, level_precision, numbering, x_true_seed=2):
     """MG-CG on the beam hierarchy (coarse 4x1x1 refined to `cells`): rhs =
     A x_true with x_true from seed 2 (constrained entries zeroed), CG to
     ||r||/||b|| <= 1e-8, Chebyshev degree 5 (range 20, 30 eigenvalue CG steps),
@@ -79,7 +81,8 @@ def mgcg_bench(mfgpu, torch, cells, p, level_precision, x_true_seed=2):
             (cells[0] >> n_levels) << n_levels == cells[0]:
         n_levels += 1
     coarse = [c >> (n_levels - 1) for c in cells]
-    meshes = [mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], [c << l for c in coarse], p, 3, 1, 7) for l in range(n_levels)]
+    meshes = [mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], [c << l for c in coarse], p, 3, 1, 7, numbering=numbering)
+              for l in range(n_levels)]
     dev = torch.device("cuda:0")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -275,11 +278,11 @@ def serving_bench(mfgpu, n_sessions=32, n_requests=64, rate=40.0, seed=100, max_
     }
 
 
-def cachings_bench(mfgpu, torch, cells, p, reps=20):
+def cachings_bench(mfgpu, torch, cells, p, numbering, reps=20):
     """The four MF cachings (same operator) on the beam: FP64 vmult time and
     the words each stores per quadrature point (the memory/arithmetic trade)."""
     dev = torch.device("cuda:0")
-    mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], cells, p, 3, 1, 7)
+    mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], cells, p, 3, 1, 7, numbering=numbering)
     u0 = torch.from_numpy(mesh.u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
     src = mfgpu.fill_uniform(1, torch.empty(mesh.n_dofs, dtype=torch.float64, device=dev))
     out, ref = {}, None
@@ -322,6 +325,8 @@ def main():
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
     ap.add_argument("--degree", type=int, default=BEAM["degree"])
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
+    ap.add_argument("--numbering", default=NUMBERING, choices=["cellwise", "lexicographic"],
+                    help="node numbering of the bench meshes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
@@ -342,7 +347,7 @@ def main():
         print(json.dumps({"serving": serving_bench(mfgpu)}), flush=True)
         return
     p = args.degree
-    mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], args.cells, p, 3, 1, 7)
+    mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], args.cells, p, 3, 1, 7, numbering=args.numbering)
     n_dof, n_cell = mesh.n_dofs, mesh.n_cells
     op = mfgpu.NeoHookOperator(mesh, MU, NU, RHO, ALPHA1, formulation=1, caching="tensor2")
     u0 = torch.from_numpy(mesh.u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
@@ -405,11 +410,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"tangent vmult, beam [0,4]x[0,1]^2, {args.cells[0]}x{args.cells[1]}x{args.cells[2]} "
                                f"cells, FE_Q({p})^3, QGauss({p + 1}), {n_dof} DoFs, u=0 on x=0, "
-                               "linearised at 1e-3 sin sin sin + 1e-5 noise",
+                               f"linearised at 1e-3 sin sin sin + 1e-5 noise, {args.numbering} node numbering",
                    "n_dofs": n_dof, "n_cells": n_cell, "degree": p, "caching": "tensor2", "formulation": 1},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p)),
+                     "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p, args.numbering)),
                      "traffic_source": "profiles/vmult_q4_beam_fp64_pmc_traffic.md",
                      "kernel": "apply_cells<4,2,double,QSolidT2F1>", "kernel_avg_ms": round(kern_avg_ms, 4),
                      "canonical_bytes_per_launch": B},
@@ -418,17 +423,17 @@ def main():
     del op, src, dst
     torch.cuda.empty_cache()
     if not args.no_mg:
-        r64, x64 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP64)
+        r64, x64 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP64, args.numbering)
         x64 = x64.cpu()
         torch.cuda.empty_cache()
-        r32, x32 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP32)
+        r32, x32 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP32, args.numbering)
         r32["err_vs_fp64_solution"] = float(torch.linalg.norm(x32.cpu() - x64) / torch.linalg.norm(x64))
         out["mgcg"] = {"workload": "MG-CG to ||r||/||b|| <= 1e-8, rhs = A x_true (seed 2), Chebyshev degree 5, "
                                    "coarse FP64 CG to 1e-3", "fp64": r64, "mixed_fp32_vcycle": r32,
                        "speedup_mixed_vs_fp64": round(r64["solve_s"] / r32["solve_s"], 3)}
     if not args.no_cachings:
         torch.cuda.empty_cache()
-        out["cachings"] = cachings_bench(mfgpu, torch, args.cells, p)
+        out["cachings"] = cachings_bench(mfgpu, torch, args.cells, p, args.numbering)
     if not args.no_batched:
         torch.cuda.empty_cache()
         out["batched"] = batched_bench(mfgpu, torch)

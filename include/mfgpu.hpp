@@ -84,8 +84,9 @@ class DeviceVector {
 class BoxLevel {
  public:
   BoxLevel(const double lo[3], const double hi[3], const int32_t n[3], int degree, int n_components, int face_mask,
-           int comp_mask) {
+           int comp_mask, int numbering = MFG_NUMBERING_LEXICOGRAPHIC) {
     check(mfg_box_mesh_create(lo, hi, n, degree, n_components, face_mask, comp_mask, &mesh_));
+    check(mfg_box_mesh_renumber(mesh_, numbering));
     check(mfg_box_mesh_describe(mesh_, &desc_));
   }
   ~BoxLevel() { mfg_box_mesh_destroy(mesh_); }

@@ -51,6 +51,7 @@ struct Session {
       mfg_mesh m = nullptr;
       if (mfg_box_mesh_create(spec.lo, spec.hi, nl, spec.degree, 3, 1, 7, &m)) throw Error(MFG_ERR_ARGUMENT, mfg_last_error());
       meshes.push_back(m);
+      if (mfg_box_mesh_renumber(m, MFG_NUMBERING_CELLWISE)) throw Error(MFG_ERR_ARGUMENT, mfg_last_error());
       mfg_level_desc d;
       mfg_box_mesh_describe(m, &d);
       descs.push_back(d);

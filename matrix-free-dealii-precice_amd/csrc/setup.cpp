@@ -140,6 +140,8 @@ struct mfg_mesh_t {
   std::vector<double> vertices;
   std::vector<int32_t> constrained;
   std::vector<double> node_x;
+  std::vector<int32_t> lex_id;  // node -> its lexicographic number (identity until renumbered)
+  int numbering = MFG_NUMBERING_LEXICOGRAPHIC;
 };
 
 using namespace mfg;
@@ -188,6 +190,8 @@ int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n[
         for (int d = 0; d < 3; ++d) m->vertices[c * 24 + v * 3 + d] = lo[d] + (cc[d] + o[d]) * h[d];
       }
     }
+    m->lex_id.resize(m->n_nodes);
+    for (int64_t v = 0; v < m->n_nodes; ++v) m->lex_id[v] = (int32_t)v;
     // node coordinates from the cell-local GLL support points
     m->node_x.resize(m->n_nodes * 3);
     std::vector<double> line[3];
@@ -249,8 +253,9 @@ int mfg_box_mesh_fill_u0(mfg_mesh m, double amp, double noise, uint64_t seed, do
       const double *x = &m->node_x[node * 3];
       const double base = amp * std::sin(M_PI * x[0]) * std::sin(M_PI * x[1]) * std::sin(M_PI * x[2]);
       for (int c = 0; c < m->nc; ++c) {
-        const int64_t dof = node * m->nc + c;
-        out[dof] = con[dof] ? 0.0 : base + noise * uniform_pm1_host(seed, (uint64_t)dof);
+        // the noise is drawn per lexicographic dof: the same field under any numbering
+        const int64_t dof = node * m->nc + c, lex_dof = (int64_t)m->lex_id[node] * m->nc + c;
+        out[dof] = con[dof] ? 0.0 : base + noise * uniform_pm1_host(seed, (uint64_t)lex_dof);
       }
     }
   });
@@ -262,6 +267,72 @@ int mfg_box_mesh_cells(mfg_mesh m, int32_t n[3]) {
     for (int d = 0; d < 3; ++d) n[d] = m->n[d];
   });
 }
+
+// Renumbers the nodes of a box mesh in place: cell_nodes, node coordinates and
+// the Dirichlet list follow the permutation; cells keep their order.
+//  - LEXICOGRAPHIC: node (I, J, K) of the global lattice is I + NN0 (J + NN1 K).
+//  - CELLWISE: nodes are numbered cell by cell, as DoFHandler::distribute_dofs
+//    does (mf_elasticity.h:667, heat_transfer.h:437), but every cell (i, j, k)
+//    owns exactly the p^3 nodes (i p + a, j p + b, k p + c), a, b, c in 1..p,
+//    so block c holds nodes [c p^3, (c + 1) p^3) and, for p = 4, starts on a
+//    128-byte line of a 3-component FP64 vector.  The nodes of the three low
+//    boundary planes follow all blocks.  Inside a block the nodes go entity by
+//    entity -- x edge, y face, y edge, vertex, z edge, x face, z face, interior
+//    (an entity named by the directions in which it is not at the cell's high
+//    side) -- and lexicographically inside an entity: the pieces a neighbouring
+//    cell reads (a face with its edges and vertex) then fall into few lines.
+int mfg_box_mesh_renumber(mfg_mesh m, int32_t numbering) {
+  return guarded([&] {
+    MFG_REQUIRE(m, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(numbering == MFG_NUMBERING_LEXICOGRAPHIC || numbering == MFG_NUMBERING_CELLWISE, MFG_ERR_ARGUMENT,
+                "unknown numbering");
+    const int p = m->p, N = p + 1, N3 = N * N * N;
+    const int64_t n_cells = (int64_t)m->n[0] * m->n[1] * m->n[2];
+    std::vector<int32_t> newid((size_t)m->n_nodes, -1);
+    if (numbering == MFG_NUMBERING_LEXICOGRAPHIC) {
+      for (int64_t c = 0; c < n_cells; ++c) {
+        const int cx = (int)(c % m->n[0]), cy = (int)((c / m->n[0]) % m->n[1]), cz = (int)(c / ((int64_t)m->n[0] * m->n[1]));
+        for (int l = 0; l < N3; ++l) {
+          const int i = l % N, j = (l / N) % N, k = l / (N * N);
+          newid[m->cell_nodes[c * N3 + l]] =
+              (int32_t)((cx * p + i) + (int64_t)m->NN[0] * ((cy * p + j) + (int64_t)m->NN[1] * (cz * p + k)));
+        }
+      }
+    } else {
+      // entity of a block node: bit d set where coordinate d is below the
+      // cell's high side; rank of each entity in the numbering order
+      static const int kRank[8] = {/*vertex*/ 3, /*x edge*/ 0, /*y edge*/ 2, /*z face*/ 6,
+                                   /*z edge*/ 4, /*y face*/ 1, /*x face*/ 5, /*interior*/ 7};
+      std::vector<int> block;
+      for (int l = 0; l < N3; ++l)
+        if (l % N > 0 && (l / N) % N > 0 && l / (N * N) > 0) block.push_back(l);
+      auto entity = [&](int l) { return (l % N < p ? 1 : 0) | ((l / N) % N < p ? 2 : 0) | (l / (N * N) < p ? 4 : 0); };
+      std::stable_sort(block.begin(), block.end(), [&](int a, int b) { return kRank[entity(a)] < kRank[entity(b)]; });
+      int32_t next = 0;
+      for (int64_t c = 0; c < n_cells; ++c)
+        for (int l : block) newid[m->cell_nodes[c * N3 + l]] = next++;
+      for (int64_t c = 0; c < n_cells; ++c)  // the low boundary planes
+        for (int l = 0; l < N3; ++l) {
+          const int32_t old = m->cell_nodes[c * N3 + l];
+          if (newid[old] < 0) newid[old] = next++;
+        }
+      MFG_REQUIRE(next == m->n_nodes, MFG_ERR_ARGUMENT, "mesh has nodes in no cell");
+    }
+    for (auto &x : m->cell_nodes) x = newid[x];
+    std::vector<double> nx(m->node_x.size());
+    for (int64_t v = 0; v < m->n_nodes; ++v)
+      for (int d = 0; d < 3; ++d) nx[(size_t)newid[v] * 3 + d] = m->node_x[(size_t)v * 3 + d];
+    m->node_x.swap(nx);
+    std::vector<int32_t> lex(m->lex_id.size());
+    for (int64_t v = 0; v < m->n_nodes; ++v) lex[newid[v]] = m->lex_id[v];
+    m->lex_id.swap(lex);
+    for (auto &dof : m->constrained) dof = (int32_t)(newid[dof / m->nc] * m->nc + dof % m->nc);
+    std::sort(m->constrained.begin(), m->constrained.end());
+    m->numbering = numbering;
+  });
+}
+
+int32_t mfg_box_mesh_numbering(mfg_mesh m) { return m ? m->numbering : -1; }
 
 int mfg_box_mesh_node_coords(mfg_mesh m, double *out) {
   return guarded([&] {

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libmfgpu.so")
 FP64, FP32 = 0, 1
 CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
 CELLS_COLOURS, CELLS_BUNDLES, CELLS_PENCILS = 0, 1, 2
+NUMBERING = {"lexicographic": 0, "cellwise": 1}
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
 _c_dp = ctypes.POINTER(ctypes.c_double)
@@ -53,6 +54,8 @@ SIGNATURES = {
     "mfg_box_mesh_destroy": (ctypes.c_int, [_vp]),
     "mfg_box_mesh_describe": (ctypes.c_int, [_vp, ctypes.POINTER(LevelDesc)]),
     "mfg_box_mesh_node_coords": (ctypes.c_int, [_vp, _c_dp]),
+    "mfg_box_mesh_renumber": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_box_mesh_numbering": (ctypes.c_int32, [_vp]),
     "mfg_solid_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), ctypes.POINTER(SolidParams), ctypes.c_int32,
                                          ctypes.POINTER(_vp)]),
     "mfg_heat_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), ctypes.c_double, _c_dp, ctypes.c_int32,
@@ -143,7 +146,7 @@ def _dtype_of(precision):
 class BoxMesh:
     """Structured box mesh level (cells, nodes, Dirichlet dofs) built on the host."""
 
-    def __init__(self, lo, hi, n_cells, degree, n_components, face_mask=1, comp_mask=7):
+    def __init__(self, lo, hi, n_cells, degree, n_components, face_mask=1, comp_mask=7, numbering="lexicographic"):
         import numpy as np
         self._lo = np.ascontiguousarray(lo, dtype=np.float64)
         self._hi = np.ascontiguousarray(hi, dtype=np.float64)
@@ -154,6 +157,10 @@ class BoxMesh:
                                          comp_mask, ctypes.byref(h)))
         self.h = h
         self._destroy = lib().mfg_box_mesh_destroy
+        if numbering not in NUMBERING:
+            raise ValueError(f"numbering must be one of {sorted(NUMBERING)}")
+        check(lib().mfg_box_mesh_renumber(self.h, NUMBERING[numbering]))
+        self.numbering = numbering
         self.desc = LevelDesc()
         check(lib().mfg_box_mesh_describe(self.h, ctypes.byref(self.desc)))
         self.degree, self.n_components = degree, n_components
@@ -720,7 +727,7 @@ class Instance:
     def __init__(self, kind, n_cells, degree, fine_precision=FP64, level_precision=FP32, lo=(0.0, 0.0, 0.0),
                  hi=(1.0, 1.0, 1.0), dt=0.1, conductivity=1.0, formulation=1, cheb_degree=5, smoothing_range=None,
                  eig_cg_iterations=None, coarse_eig_iterations=0, amp=1e-3, noise=1e-5, seed=0,
-                 material=(384615.38461538462, 0.3, 1000.0, 4e6), cell_schedule=None):
+                 material=(384615.38461538462, 0.3, 1000.0, 4e6), cell_schedule=None, numbering="cellwise"):
         import numpy as np
         import torch
         self.kind, self.degree = kind, degree
@@ -729,7 +736,7 @@ class Instance:
             counts.insert(0, [c // 2 for c in counts[0]])
         solid = kind == "solid"
         nc, fm, cm = (3, 1, 7) if solid else (1, 63, 1)
-        self.meshes = [BoxMesh(lo, hi, c, degree, nc, fm, cm) for c in counts]
+        self.meshes = [BoxMesh(lo, hi, c, degree, nc, fm, cm, numbering=numbering) for c in counts]
         fine_mesh = self.meshes[-1]
         mu, nu, rho, alpha_1 = material
         if solid:
