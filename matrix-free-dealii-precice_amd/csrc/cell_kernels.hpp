@@ -93,6 +93,17 @@ struct Tile {
   }
 };
 
+// The gradient phase of evaluate_vg and the first phase of integrate_vg read
+// 3 NC lines of N LDS values each.  Unrolled over the components, the
+// compiler issues all of them at once: for N >= 5 that peak (about 120 VGPRs
+// in FP64) holds the cell kernel at 2 waves per SIMD.  A rolled component
+// loop frees those registers (3 to 4 waves); for N <= 4 the unrolled form is
+// faster.
+template <int N>
+constexpr bool rolled_components() {
+  return N >= 5;
+}
+
 // Evaluate values and reference-coordinate gradients at the quadrature points
 // of NC components; slots 0..2NC-1 are used.  On return the B slots hold the
 // point values (still read by later lines until the next barrier).
@@ -118,11 +129,17 @@ __device__ inline void evaluate_vg(const Tile<N, Number> &T, const Number *S, co
 #pragma unroll
   for (int c = 0; c < NC; ++c) T.put(NC + c, val[c]);
   __syncthreads();
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
+  auto gradient = [&](int c) {
     grad[c][0] = T.template line<0>(D, T.i, NC + c);
     grad[c][1] = T.template line<1>(D, T.j, NC + c);
     grad[c][2] = T.template line<2>(D, T.k, NC + c);
+  };
+  if constexpr (rolled_components<N>()) {
+#pragma unroll 1
+    for (int c = 0; c < NC; ++c) gradient(c);
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) gradient(c);
   }
 }
 
@@ -138,10 +155,17 @@ __device__ inline void integrate_vg(const Tile<N, Number> &T, const Number *S, c
     for (int f = 0; f < 3; ++f) T.put(3 * c + f, grad[c][f]);
   __syncthreads();
   Number r[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
+  auto gradient_t = [&](int c) {
     r[c] = val[c] + T.template line_t<0>(D, T.i, 3 * c + 0) + T.template line_t<1>(D, T.j, 3 * c + 1) +
            T.template line_t<2>(D, T.k, 3 * c + 2);
+  };
+  if constexpr (rolled_components<N>()) {
+#pragma unroll 1
+    for (int c = 0; c < NC; ++c) gradient_t(c);
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) gradient_t(c);
+  }
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; ++c) T.put(c, r[c]);
@@ -610,15 +634,22 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
     const bool skip = !active || ((entry >> (kConstraintShift + c)) & 1u);
     u[c] = skip ? Number(0) : src[node * NC + c];
   }
-  // dst of this node, read now: no other workgroup of the launch writes it,
-  // so the scatter at the end is a plain store and this load's latency is
-  // hidden behind the sweeps.  The first colour touching a node stores.
+  // dst of this node: no other workgroup of the launch writes it, so the
+  // scatter at the end is a plain store.  The first colour touching a node
+  // stores without reading.  Read with the gather, the load's latency hides
+  // behind the sweeps; read after them, its 2 NC registers are free during
+  // the sweeps.  For FP64 and p <= 4 the late read gives the kernel one more
+  // wave per SIMD and is the faster of the two (measured); elsewhere the
+  // early read is.
+  constexpr bool kLateDst = sizeof(Number) == 8 && P <= 4;
   const bool first = assign && (entry >> 31);
   Number dold[NC];
+  if constexpr (!kLateDst) {
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const bool skip = !active || first || ((entry >> (kConstraintShift + c)) & 1u);
-    dold[c] = skip ? Number(0) : dst[node * NC + c];
+    for (int c = 0; c < NC; ++c) {
+      const bool skip = !active || first || ((entry >> (kConstraintShift + c)) & 1u);
+      dold[c] = skip ? Number(0) : dst[node * NC + c];
+    }
   }
   typename QOp::Data qd = qop.load(qidx);
   __syncthreads();
@@ -637,6 +668,13 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   qop.apply(qd, val, grad);
   Number out[NC];
   integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
+  if constexpr (kLateDst) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const bool skip = !active || first || ((entry >> (kConstraintShift + c)) & 1u);
+      dold[c] = skip ? Number(0) : dst[node * NC + c];
+    }
+  }
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
