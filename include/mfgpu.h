@@ -41,8 +41,10 @@ extern "C" {
  * mf_nh_operator.h:346-523, 848-1010).  All four define the same operator:
  *   scalar_referential: reference geometry + 1 scalar per point; F rebuilt
  *                       from the cached nodal displacement at every vmult
- *   tensor2:            deformed geometry + 2 scalars + tau/J (18 words/point;
- *                       the data the canonical roofline counts)
+ *   tensor2:            deformed geometry + 2 scalars + tau/J (the reference
+ *                       stores 18 words/point, the data the canonical roofline
+ *                       counts; formulation 1 stores them folded into 16,
+ *                       or 17 on meshes with non-affine cells)
  *   tensor4:            tensor2 data + the symmetric 4th-order tangent (53 words)
  *   tensor4_ns:         reference geometry + the full 4th-order tangent (91 words)
  * Formulation 0 exists for tensor2 only (mf_nh_operator.h:706). */
@@ -159,8 +161,11 @@ int mfg_operator_precondition_jacobi(mfg_operator op, void *dst, const void *src
 size_t mfg_operator_memory_consumption(mfg_operator op);
 /* Quadrature-point cache as stored on the device (structure of arrays,
  * field-major: field f of point (cell, q) at f*n_cells*nq3 + cell*nq3 + q).
- * Solid tensor2 formulation 1: 18 fields  Jinv_cur[f][e] (9), JxW_cur,
- * 2(mu-2 lambda ln J)/J, 1/J, tau/J (xx,yy,zz,xy,xz,yz).  Heat: 11 fields
+ * Solid tensor2 formulation 1: Jinv_cur[f][e] (9), JxW_ref s, JxW_ref tau
+ * (xx,yy,zz,xy,xz,yz), with s = mu - 2 lambda ln J and JxW_ref = JxW_cur / J
+ * (the reference's JxW_cur 2s/J / 2 and JxW_cur tau/J); a 17th field JxW_ref
+ * when some cell is not affine in the reference configuration (otherwise it is
+ * a per-cell determinant times the quadrature weight).  Heat: 11 fields
  * Jinv (9), JxW, c.  Copies n_fields*n_qp values to the device pointer out;
  * n_fields may be queried with out == NULL. */
 int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields);

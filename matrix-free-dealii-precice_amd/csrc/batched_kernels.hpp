@@ -45,6 +45,7 @@ struct BatchInst {
   int code, cpb;
   int id;  // batch-wide instance number (active mask)
   Number prm[3];  // F1: 2 lambda, rho alpha_1; F0: kappa, c_1, rho alpha_1; heat: dt
+  const Number *cell_jxw, *wq;  // F1: per-cell reference JxW (or null) and tensor weights
 };
 
 struct BatchTask {
@@ -100,7 +101,7 @@ __device__ inline void batch_task_degree(int kind, const BatchInst<Number> &in, 
                                          const BatchTask &tk, int assign, Number *s_tab, Number *s_tile) {
   if constexpr ((KINDS & kind_bit(kSolidF1)) != 0) {
     if (kind == kSolidF1) {
-      QSolidT2F1<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1]};
+      QSolidT2F1<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1], in.cell_jxw, in.wq};
       batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
     }
   }

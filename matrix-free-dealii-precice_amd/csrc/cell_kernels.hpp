@@ -201,33 +201,47 @@ __device__ inline Number stream_load(const Number *p) {
 }
 
 // ------------------------------------------------------ quadrature ops ---
-// Each op provides load(qidx) -> Data and apply(Data, val, grad), turning
-// values/reference gradients at a point into submitted value and reference
-// gradient already multiplied by JxW.
+// Each op provides load(qidx, cell, pt) -> Data (point qidx = cell N^3 + pt
+// of the stored cell order) and apply(Data, val, grad), turning values and
+// reference gradients at a point into submitted value and reference gradient
+// already multiplied by JxW.
 
 // Solid, tensor2 caching, formulation 1 (mf_nh_operator.h:945-974).
-// Fields: Jinv_cur[f][e] (9), JxW, c2 = 2(mu-2 lambda lnJ)/J, c1 = 1/J,
-// T = tau/J (xx,yy,zz,xy,xz,yz).
+// The reference caches, per point, the current-frame Jinv and JxW (from
+// MappingQEulerian) and the scalars 2 s / J, 1 / J and tau / J, where
+// s = mu - 2 lambda ln J and tau is the Kirchhoff stress.  The operator only
+// uses them multiplied by the current JxW = JxW_ref J, so these products are
+// cached instead:
+//   fields 0..8   Jinv_cur[f][e]
+//   field  9      JxW_ref s
+//   fields 10..15 JxW_ref tau (xx, yy, zz, xy, xz, yz)
+//   field  16     JxW_ref, only on meshes with a cell that is not affine in the
+//                 reference configuration.  Otherwise JxW_ref = w_q det J_ref
+//                 comes from one word per cell (cell_jxw) and the tensor
+//                 weights (wq).
+// 16 words per point instead of the reference's 18, and no JxW products in
+// the point operation.
 template <typename Number>
 struct QSolidT2F1 {
   static constexpr int NC = 3;
-  static constexpr int kFields = 18;
+  static constexpr int kFields = 17;  // 16 with cell_jxw
   static constexpr bool kNodalField = false;
   const Number *qd;
   int64_t nqp;
   Number two_lambda, rho_alpha;
+  const Number *cell_jxw;  // det J_ref per stored cell, or null: field 16 holds JxW_ref
+  const Number *wq;        // N^3 tensor-product quadrature weights (with cell_jxw)
   struct Data {
-    Number ji[9], jxw, c2, c1, T[6];
+    Number ji[9], s, T[6], w;
   };
-  __device__ Data load(int64_t q) const {
+  __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
-    d.jxw = stream_load(qd + 9 * nqp + q);
-    d.c2 = stream_load(qd + 10 * nqp + q);
-    d.c1 = stream_load(qd + 11 * nqp + q);
+    d.s = stream_load(qd + 9 * nqp + q);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) d.T[f] = stream_load(qd + (12 + f) * nqp + q);
+    for (int f = 0; f < 6; ++f) d.T[f] = stream_load(qd + (10 + f) * nqp + q);
+    d.w = cell_jxw ? cell_jxw[cell] * wq[pt] : stream_load(qd + 16 * nqp + q);
     return d;
   }
   __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
@@ -237,8 +251,7 @@ struct QSolidT2F1 {
 #pragma unroll
       for (int e = 0; e < 3; ++e) g[c][e] = grad[c][0] * d.ji[0 * 3 + e] + grad[c][1] * d.ji[1 * 3 + e] + grad[c][2] * d.ji[2 * 3 + e];
     const Number tr = g[0][0] + g[1][1] + g[2][2];
-    const Number diag = two_lambda * d.c1 * tr;
-    const Number half_c2 = Number(0.5) * d.c2;
+    const Number diag = two_lambda * d.w * tr;
     // T symmetric: T[0]=xx T[1]=yy T[2]=zz T[3]=xy T[4]=xz T[5]=yz
     const Number Tm[3][3] = {{d.T[0], d.T[3], d.T[4]}, {d.T[3], d.T[1], d.T[5]}, {d.T[4], d.T[5], d.T[2]}};
     Number Pm[3][3];
@@ -246,14 +259,14 @@ struct QSolidT2F1 {
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = 0; b < 3; ++b)
-        Pm[a][b] = half_c2 * (g[a][b] + g[b][a]) + (a == b ? diag : Number(0)) + g[a][0] * Tm[0][b] +
+        Pm[a][b] = d.s * (g[a][b] + g[b][a]) + (a == b ? diag : Number(0)) + g[a][0] * Tm[0][b] +
                    g[a][1] * Tm[1][b] + g[a][2] * Tm[2][b];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int f = 0; f < 3; ++f)
-        grad[c][f] = d.jxw * (Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2]);
-      val[c] = d.jxw * rho_alpha * d.c1 * val[c];
+        grad[c][f] = Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2];
+      val[c] = rho_alpha * d.w * val[c];
     }
   }
 };
@@ -271,7 +284,7 @@ struct QSolidT2F0 {
   struct Data {
     Number ji[9], jxw, jm13, F[9];
   };
-  __device__ Data load(int64_t q) const {
+  __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
@@ -358,7 +371,7 @@ struct QHeat {
   struct Data {
     Number ji[9], jxw, c;
   };
-  __device__ Data load(int64_t q) const {
+  __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
@@ -397,7 +410,7 @@ struct QSolidSR {
   struct Data {
     Number ji[9], jxw, sc, ug[9];
   };
-  __device__ Data load(int64_t q) const {
+  __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
@@ -483,7 +496,7 @@ struct QSolidT4 {
     Number ji[9], jxw, c1, T[6];
     int64_t q;
   };
-  __device__ Data load(int64_t q) const {
+  __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
@@ -545,7 +558,7 @@ struct QSolidT4NS {
     Number ji[9], jxw;
     int64_t q;
   };
-  __device__ Data load(int64_t q) const {
+  __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
@@ -651,7 +664,7 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
       dold[c] = skip ? Number(0) : dst[node * NC + c];
     }
   }
-  typename QOp::Data qd = qop.load(qidx);
+  typename QOp::Data qd = qop.load(qidx, active ? cell : 0, pt);
   __syncthreads();
   if constexpr (QOp::kNodalField) {
     // reference gradient of the cached nodal field (read plain: constrained
@@ -794,7 +807,7 @@ __global__ void __launch_bounds__((bundle_threads<P, BW>()))
       u[c] = (!active || con) ? Number(0) : a.src[node * NC + c];
       dold[c] = (!storing || first || con) ? Number(0) : a.dst[node * NC + c];
     }
-    typename QOp::Data qd = qop.load(qidx);
+    typename QOp::Data qd = qop.load(qidx, active ? cell : 0, pt);
     __syncthreads();  // tables visible; the previous step's LDS reads are done
     if constexpr (QOp::kNodalField) {
       Number w[3], wg[3][3];

@@ -22,6 +22,13 @@ struct DefaultCPB {
   static constexpr int value = P == 1 ? 8 : P == 2 ? 7 : P == 6 ? 1 : 2;
 };
 
+// per-cell reference JxW of the tensor2 formulation-1 op, or null when it is
+// a per-point field
+template <typename Number>
+const Number *t2_cell_jxw(const OperatorData &op) {
+  return op.affine_ref ? op.cell_jxw.as<Number>() : nullptr;
+}
+
 template <typename Number>
 static void fill(Number *x, int64_t n, Number v, hipStream_t s) {
   if (n <= 0) return;
@@ -117,7 +124,8 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
       if (!launch_bundles(q))
         launch_simple(apply_cells<P, CPB, Number, QSolidT4NS<Number>>, q);
     } else if (op.sp.formulation == 1) {
-      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), ra};
+      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), ra, t2_cell_jxw<Number>(op),
+                           op.wq3.as<Number>()};
       if (!launch_bundles(q))
         launch_simple(apply_cells<P, CPB, Number, QSolidT2F1<Number>>, q);
     } else {
@@ -151,6 +159,7 @@ static void launch_cache(OperatorData &op, const void *u, int u_fp32) {
   a.lambda = Number(op.lambda);
   a.formulation = op.sp.formulation;
   a.caching = op.sp.caching;
+  a.jxw_ref_field = op.affine_ref ? 0 : 1;
   a.error = op.flag.as<int>();
   const int grid = blocks_for(op.n_cells, CPB);
   cache_solid<P, CPB, Number><<<grid, block_threads<P, CPB>(), 0, op.stream>>>(a);
@@ -183,7 +192,8 @@ static void launch_diagonal(OperatorData &op) {
         QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
         diagonal_cells<P, Number, QSolidT4NS<Number>><<<grid, NT, 0, op.stream>>>(a, q);
       } else if (op.sp.formulation == 1) {
-        QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha)};
+        QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha),
+                             t2_cell_jxw<Number>(op), op.wq3.as<Number>()};
         diagonal_cells<P, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
       } else {
         QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
@@ -565,6 +575,29 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
     }
     for (int x = 0; x < 24; ++x) vert[(size_t)c * 24 + x] = d.cell_vertices[o * 24 + x];
   }
+  // affine cells: the mixed terms of the trilinear map vanish (vertex v at
+  // (v & 1, v >> 1 & 1, v >> 2 & 1)); then det J_ref is one number per cell
+  std::vector<double> det_ref((size_t)n_cells);
+  affine_ref = true;
+  for (int c = 0; c < n_cells; ++c) {
+    const double *X = &vert[(size_t)c * 24];
+    auto V = [&](int v, int e) { return X[v * 3 + e]; };
+    double J[3][3], h = 0, mixed = 0;
+    for (int e = 0; e < 3; ++e) {
+      J[e][0] = V(1, e) - V(0, e);
+      J[e][1] = V(2, e) - V(0, e);
+      J[e][2] = V(4, e) - V(0, e);
+      for (int f = 0; f < 3; ++f) h = std::max(h, std::fabs(J[e][f]));
+      mixed = std::max({mixed, std::fabs(V(3, e) - V(2, e) - V(1, e) + V(0, e)),
+                        std::fabs(V(5, e) - V(4, e) - V(1, e) + V(0, e)),
+                        std::fabs(V(6, e) - V(4, e) - V(2, e) + V(0, e)),
+                        std::fabs(V(7, e) - V(6, e) - V(5, e) + V(4, e) - V(3, e) + V(2, e) + V(1, e) - V(0, e))});
+    }
+    if (mixed > 1e-12 * h) affine_ref = false;
+    det_ref[c] = std::fabs(J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
+  }
   upload(cell_nodes, map.data(), map.size(), stream);
   upload(constrained, d.constrained_dofs, (size_t)n_constrained, stream);
   host_constrained.assign(d.constrained_dofs, d.constrained_dofs + n_constrained);
@@ -576,6 +609,19 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
   const Tables1D t = make_tables(p);
   if (precision == MFG_FP64) upload_tables<double>(*this, t);
   else upload_tables<float>(*this, t);
+  if (kind == 0 && affine_ref) {
+    const int N = p + 1;
+    std::vector<double> w3(N3);
+    for (int x = 0; x < N3; ++x) w3[x] = t.weight[x % N] * t.weight[(x / N) % N] * t.weight[x / (N * N)];
+    if (precision == MFG_FP64) {
+      upload(cell_jxw, det_ref.data(), det_ref.size(), stream);
+      upload(wq3, w3.data(), w3.size(), stream);
+    } else {
+      std::vector<float> a(det_ref.begin(), det_ref.end()), b(w3.begin(), w3.end());
+      upload(cell_jxw, a.data(), a.size(), stream);
+      upload(wq3, b.data(), b.size(), stream);
+    }
+  }
 }
 
 template <typename Number>
@@ -637,14 +683,17 @@ void OperatorData::setup_solid() {
     case MFG_CACHING_SCALAR_REFERENTIAL: n_fields = QSolidSR<double>::kFields; break;
     case MFG_CACHING_TENSOR4: n_fields = QSolidT4<double>::kFields; break;
     case MFG_CACHING_TENSOR4_NS: n_fields = QSolidT4NS<double>::kFields; break;
-    default: n_fields = sp.formulation == 1 ? QSolidT2F1<double>::kFields : QSolidT2F0<double>::kFields;
+    default:
+      n_fields = sp.formulation == 0 ? QSolidT2F0<double>::kFields
+                 : affine_ref        ? QSolidT2F1<double>::kFields - 1
+                                     : QSolidT2F1<double>::kFields;
   }
   const size_t es = precision == MFG_FP64 ? 8 : 4;
   qd.allocate((size_t)n_fields * nqp * es);
 }
 
 size_t OperatorData::memory_consumption() const {
-  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes;
+  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes + cell_jxw.bytes + wq3.bytes;
 }
 
 }  // namespace mfg

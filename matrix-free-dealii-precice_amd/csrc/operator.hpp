@@ -21,6 +21,10 @@ struct OperatorData {
   DeviceBuffer qd;           // quadrature cache, field-major SoA
   DeviceBuffer diag, inv_diag;
   DeviceBuffer u_nodal;      // scalar_referential caching: the linearisation point
+  // reference-frame JxW per cell: when every cell's trilinear map is affine,
+  // JxW_ref = cell_jxw[stored cell] * wq3[point] (tensor2, formulation 1)
+  bool affine_ref = false;
+  DeviceBuffer cell_jxw, wq3;
   DeviceBuffer flag;         // int error flag
   std::vector<int32_t> host_constrained;
   std::vector<int32_t> host_cell_nodes;  // input cell order (used to build transfers)

@@ -21,6 +21,7 @@ struct CacheArgs {
   Number mu, lambda;
   int formulation;
   int caching;  // MFG_CACHING_*
+  int jxw_ref_field;  // tensor2 formulation 1: store JxW_ref as field 16 (not supplied per cell)
   int *error;   // set to 1 if det F <= 0 anywhere
 };
 
@@ -169,9 +170,9 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
   const Number jxw = (Number)wq * fabs(det3(Jc));
 #pragma unroll
   for (int f = 0; f < 9; ++f) a.qd[f * n + qidx] = Jci[f / 3][f % 3];
-  a.qd[9 * n + qidx] = jxw;
   if (a.formulation == 0) {
     // mf_nh_operator.h:419-440: J^{-1/3} and F
+    a.qd[9 * n + qidx] = jxw;
     a.qd[10 * n + qidx] = pow(J, Number(-1.0 / 3.0));
 #pragma unroll
     for (int f = 0; f < 9; ++f) a.qd[(11 + f) * n + qidx] = F[f / 3][f % 3];
@@ -184,6 +185,7 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
 #pragma unroll
     for (int y = 0; y < 3; ++y) b[x][y] = F[x][0] * F[y][0] + F[x][1] * F[y][1] + F[x][2] * F[y][2];
   if (a.caching == MFG_CACHING_TENSOR4) {  // mf_nh_operator.h:478-494
+    a.qd[9 * n + qidx] = jxw;
     a.qd[10 * n + qidx] = invJ;
 #pragma unroll
     for (int s6 = 0; s6 < 6; ++s6) {
@@ -202,14 +204,20 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
       }
     return;
   }
-  // tensor2, mf_nh_operator.h:456-477
-  a.qd[10 * n + qidx] = scalar * Number(2) / J;
-  a.qd[11 * n + qidx] = invJ;
+  // tensor2, formulation 1 (mf_nh_operator.h:456-477): the reference's
+  // 2 s / J and tau / J times the current JxW = JxW_ref J, i.e. JxW_ref s and
+  // JxW_ref tau, plus JxW_ref itself unless the operator takes it per cell
+  const double detr = Jr[0][0] * (Jr[1][1] * Jr[2][2] - Jr[1][2] * Jr[2][1]) -
+                      Jr[0][1] * (Jr[1][0] * Jr[2][2] - Jr[1][2] * Jr[2][0]) +
+                      Jr[0][2] * (Jr[1][0] * Jr[2][1] - Jr[1][1] * Jr[2][0]);
+  const Number jxw_ref = (Number)(wq * fabs(detr));
+  a.qd[9 * n + qidx] = jxw_ref * scalar;
 #pragma unroll
   for (int s6 = 0; s6 < 6; ++s6) {
     const Number tau = a.mu * b[kSymI[s6]][kSymJ[s6]] - (kSymI[s6] == kSymJ[s6] ? scalar : Number(0));
-    a.qd[(12 + s6) * n + qidx] = tau / J;
+    a.qd[(10 + s6) * n + qidx] = jxw_ref * tau;
   }
+  if (a.jxw_ref_field) a.qd[16 * n + qidx] = jxw_ref;
 }
 
 // ---------------------------------------------------------- diagonal ---
@@ -249,7 +257,7 @@ __global__ void __launch_bounds__(((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64) 
   const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
   Number coef[NTERM];
   {
-    typename QOp::Data qd = qop.load(qidx);
+    typename QOp::Data qd = qop.load(qidx, active ? cell : 0, pt);
     if constexpr (QOp::kNodalField) {
       __shared__ Number s_shape[N * N], s_colloc[N * N];
       if (t < N * N) {

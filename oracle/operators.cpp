@@ -707,6 +707,28 @@ void orc_mesh_cell_vertices(void *mp, double *out) {
   std::memcpy(out, m.vert.data(), m.vert.size() * sizeof(double));
 }
 
+// Replaces the cell vertices (n_cells*8*3) and moves every node to the image
+// of its cell-local Gauss-Lobatto point under the cell's trilinear map, so the
+// support points stay the nodal interpolant of that map.  A conforming vertex
+// field (shared vertices equal) gives shared nodes one image.
+void orc_mesh_set_cell_vertices(void *mp, const double *v) {
+  Mesh &m = *(Mesh *)mp;
+  std::memcpy(m.vert.data(), v, m.vert.size() * sizeof(double));
+  const int N = m.N, N3 = N * N * N;
+  for (int c = 0; c < m.n_cells; ++c)
+    for (int l = 0; l < N3; ++l) {
+      const double xi[3] = {m.gll[l % N], m.gll[(l / N) % N], m.gll[l / (N * N)]};
+      double x[3] = {0, 0, 0};
+      for (int w = 0; w < 8; ++w) {
+        double phi = 1;
+        for (int d = 0; d < 3; ++d) phi *= ((w >> d) & 1) ? xi[d] : 1 - xi[d];
+        for (int e = 0; e < 3; ++e) x[e] += phi * v[(size_t)c * 24 + w * 3 + e];
+      }
+      const int64_t node = m.cell_nodes[(size_t)c * N3 + l];
+      for (int e = 0; e < 3; ++e) m.xnode[node * 3 + e] = x[e];
+    }
+}
+
 void orc_fill_u0(void *mp, double amp, double noise, uint64_t seed, double *u) {
   Mesh &m = *(Mesh *)mp;
   for (int64_t node = 0; node < m.n_nodes; ++node) {

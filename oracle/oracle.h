@@ -53,6 +53,9 @@ void orc_mesh_cell_nodes(void *m, int32_t *out);    /* n_cells*(p+1)^3 */
 void orc_mesh_constrained(void *m, int32_t *out);   /* sorted dof list */
 void orc_mesh_node_coords(void *m, double *out);    /* n_nodes*3 */
 void orc_mesh_cell_vertices(void *m, double *out);  /* n_cells*8*3 */
+/* new cell vertices; nodes move to the trilinear images of their GLL points
+ * (call before creating operators on the mesh) */
+void orc_mesh_set_cell_vertices(void *m, const double *v);
 
 /* Nodal interpolation of the function u0(x)=amp*sin(pi x)sin(pi y)sin(pi z)
  * per component plus noise*U[-1,1] from splitmix64(seed, dof); constrained

@@ -41,6 +41,7 @@ def lib():
             "orc_mesh_constrained": (None, [_vp, _ip]),
             "orc_mesh_node_coords": (None, [_vp, _dp]),
             "orc_mesh_cell_vertices": (None, [_vp, _dp]),
+            "orc_mesh_set_cell_vertices": (None, [_vp, _dp]),
             "orc_fill_u0": (None, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _dp]),
             "orc_fill_uniform": (None, [ctypes.c_uint64, ctypes.c_int64, _dp]),
             "orc_solid_create": (_vp, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -161,6 +162,12 @@ class Mesh:
         out = np.zeros(self.n_cells * 24)
         lib().orc_mesh_cell_vertices(self.h, dptr(out))
         return out
+
+    def set_cell_vertices(self, v):
+        """New cell vertices (n_cells*8*3); nodes move to the trilinear images of their GLL points."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.n_cells * 24
+        lib().orc_mesh_set_cell_vertices(self.h, dptr(v))
 
     def u0(self, amp=1e-3, noise=1e-5, seed=0):
         u = np.zeros(self.n_dofs)

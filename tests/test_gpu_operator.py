@@ -49,14 +49,77 @@ def test_solid_vmult_fp64_all_degrees(cuda, p):
     assert np.array_equal(y.cpu().numpy()[con], x[con])
 
 
+def folded_tensor2(ref, with_jxw_ref):
+    """The device's tensor2 (formulation 1) fields from the reference's 18:
+    Jinv_cur, JxW c2 / 2, JxW T (and JxW c1 = JxW_ref on non-affine meshes)."""
+    jxw, c2, c1, T = ref[9], ref[10], ref[11], ref[12:18]
+    out = list(ref[:9]) + [jxw * c2 / 2] + [jxw * t for t in T]
+    return out + [jxw * c1] if with_jxw_ref else out
+
+
 @pytest.mark.parametrize("p", [2, 4])
 def test_solid_cache_matches_oracle(cuda, p):
     gm, om, go, oo, u = solid_pair([0, 0, 0], [1.0, 0.6, 0.4], [2, 2, 1], p, amp=1e-1)
+    assert go.n_fields() == 16  # box cells are affine: JxW_ref comes per cell
     qd = go.qp_data().cpu().numpy()
-    ref = oo.tensor2_data()
-    for f in range(18):
+    ref = folded_tensor2(oo.tensor2_data(), False)
+    for f in range(16):
         scale = np.abs(ref[f]).max()
         assert np.abs(qd[f] - ref[f]).max() <= 1e-12 * scale, f
+
+
+class DistortedMesh:
+    """A box mesh's level description with other cell vertices."""
+
+    def __init__(self, base, vertices):
+        self.base, self.vert = base, np.ascontiguousarray(vertices, dtype=np.float64)
+        self.cn, self.con = base.cell_nodes(), base.constrained()
+        d = mfgpu.LevelDesc()
+        d.degree, d.n_q_1d, d.n_components = base.degree, base.degree + 1, base.n_components
+        d.n_cells, d.n_nodes = base.n_cells, base.n_nodes
+        d.cell_nodes = self.cn.ctypes.data_as(mfgpu._c_i32p)
+        d.cell_vertices = self.vert.ctypes.data_as(mfgpu._c_dp)
+        d.constrained_dofs = self.con.ctypes.data_as(mfgpu._c_i32p)
+        d.n_constrained = len(self.con)
+        self.desc = d
+        self.n_cells, self.n_nodes, self.n_dofs, self.degree = base.n_cells, base.n_nodes, base.n_dofs, base.degree
+
+
+def distort(v, eps=0.08):
+    v = v.reshape(-1, 3)
+    x, y, z = v[:, 0], v[:, 1], v[:, 2]
+    return (v + eps * np.stack([np.sin(np.pi * y) * z, np.sin(np.pi * x) * z, x * y], axis=1)).ravel()
+
+
+@pytest.mark.parametrize("p,precision", [(2, mfgpu.FP64), (4, mfgpu.FP64), (3, mfgpu.FP32)])
+def test_solid_tensor2_non_affine_cells(cuda, p, precision):
+    """Vertices moved by a smooth map: no cell is affine, the tensor2 cache
+    keeps JxW_ref per point (17 fields); vmult, diagonal and cache agree with
+    the oracle on the same distorted geometry."""
+    print(f"non-affine tensor2 p={p} precision={precision}", flush=True)
+    lo, hi, n = [0, 0, 0], [1.0, 0.6, 0.4], [3, 2, 2]
+    om = O.Mesh(lo, hi, n, p, 3, 1, 7)
+    v = distort(om.cell_vertices())
+    om.set_cell_vertices(v)
+    gm = DistortedMesh(mfgpu.BoxMesh(lo, hi, n, p, 3, 1, 7), v)
+    u = om.u0(amp=5e-2, noise=1e-3)
+    oo = O.Solid(om, caching=1)
+    oo.cache(u)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, precision=precision)
+    go.cache(dev(u, go.dtype))
+    assert go.n_fields() == 17
+    x = O.fill_uniform(5, om.n_dofs)
+    y = torch.empty(om.n_dofs, dtype=go.dtype, device="cuda")
+    go.vmult(y, dev(x, go.dtype))
+    tol = TOL64 if precision == mfgpu.FP64 else TOL32
+    assert rel(y.double(), oo.vmult(x)) < tol
+    go.compute_diagonal()
+    assert rel(go.diagonal().double(), oo.diagonal()) < tol
+    if precision == mfgpu.FP64:
+        qd = go.qp_data().cpu().numpy()
+        ref = folded_tensor2(oo.tensor2_data(), True)
+        for f in range(17):
+            assert np.abs(qd[f] - ref[f]).max() <= 1e-12 * np.abs(ref[f]).max(), f
 
 
 @pytest.mark.parametrize("p", [1, 2, 4])
@@ -111,7 +174,7 @@ def test_vmult_add_tvmult_jacobi(cuda):
     go.precondition_Jacobi(j, dev(x), 0.65)
     assert rel(j, 0.65 * x / oo.diagonal()) < 1e-14
     assert go.m() == go.n() == om.n_dofs
-    assert go.memory_consumption() >= 18 * 8 * om.n_cells * 27
+    assert go.memory_consumption() >= go.n_fields() * 8 * om.n_cells * 27
 
 
 @pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])

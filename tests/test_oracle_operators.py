@@ -38,6 +38,46 @@ def test_cachings_define_the_same_operator():
         assert np.linalg.norm(y - ys[0]) / np.linalg.norm(ys[0]) < 1e-13
 
 
+def distorted(m, eps=0.08):
+    """The mesh with its vertices moved by a smooth map: cells are no longer affine."""
+    v = m.cell_vertices().reshape(-1, 3)
+    x, y, z = v[:, 0], v[:, 1], v[:, 2]
+    w = v + eps * np.stack([np.sin(np.pi * y) * z, np.sin(np.pi * x) * z, x * y], axis=1)
+    m.set_cell_vertices(w.ravel())
+    return m
+
+
+def test_distorted_mesh_nodes_follow_the_vertices():
+    m = distorted(O.Mesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3))
+    cn = m.cell_nodes().reshape(-1, 27)
+    x = m.node_coords()
+    v = m.cell_vertices().reshape(-1, 8, 3)
+    corner = [0, 2, 6, 8, 18, 20, 24, 26]  # local index of vertex i + 2j + 4k
+    assert np.abs(x[cn[:, corner]] - v).max() == 0.0
+    # an edge midpoint is the mean of its two vertices (trilinear map restricted to an edge)
+    assert np.abs(x[cn[:, 1]] - 0.5 * (v[:, 0] + v[:, 1])).max() < 1e-15
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_distorted_mesh_sumfact_and_cachings(p):
+    """Non-affine cells: sum factorisation equals the dense quadrature, and the
+    four cachings (reference-frame geometry from the vertices, deformed-frame
+    geometry from the displaced support points) define the same operator."""
+    m = distorted(O.Mesh([0, 0, 0], [1.0, 0.7, 0.5], [2, 2, 1], p, 3))
+    u = m.u0(amp=5e-2, noise=1e-3)
+    x = O.fill_uniform(1, m.n_dofs)
+    ys = []
+    for caching in (0, 1, 2, 3):
+        s = O.Solid(m, caching=caching)
+        s.cache(u)
+        ys.append(s.vmult(x))
+        if caching == 1:
+            yb = s.vmult_bruteforce(x)
+            assert np.linalg.norm(ys[-1] - yb) / np.linalg.norm(yb) < 1e-13
+    for y in ys[1:]:
+        assert np.linalg.norm(y - ys[0]) / np.linalg.norm(ys[0]) < 1e-12
+
+
 @pytest.mark.parametrize("form", [0, 1])
 def test_tangent_is_symmetric(form):
     m = unit_mesh(2, 2, 3)
