@@ -115,7 +115,8 @@ struct ApplyGroup {
       e.cpb = tu.cpb_for(e.code);
       e.id = ids[i];
       e.prm[0] = e.prm[1] = e.prm[2] = Number(0);
-      e.cell_jxw = op.affine_ref ? op.cell_jxw.as<Number>() : nullptr;
+      e.cell_jxw = op.affine_ref && op.kind == 0 ? op.cell_jxw.as<Number>() : nullptr;
+      e.cell_geo = op.affine_ref && op.kind == 1 ? op.cell_geo.as<Number>() : nullptr;
       e.wq = op.wq3.as<Number>();
       if (op.kind == 1) {
         e.prm[0] = Number(op.dt);

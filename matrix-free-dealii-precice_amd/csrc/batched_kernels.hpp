@@ -46,6 +46,7 @@ struct BatchInst {
   int id;  // batch-wide instance number (active mask)
   Number prm[3];  // F1: 2 lambda, rho alpha_1; F0: kappa, c_1, rho alpha_1; heat: dt
   const Number *cell_jxw, *wq;  // F1: per-cell reference JxW (or null) and tensor weights
+  const Number *cell_geo;       // heat: per-cell Jinv and det J (or null)
 };
 
 struct BatchTask {
@@ -113,7 +114,7 @@ __device__ inline void batch_task_degree(int kind, const BatchInst<Number> &in, 
   }
   if constexpr ((KINDS & kind_bit(kHeat)) != 0) {
     if (kind == kHeat) {
-      QHeat<Number> q{in.qd, in.nqp, in.prm[0]};
+      QHeat<Number> q{in.qd, in.nqp, in.prm[0], in.cell_geo, in.wq};
       batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
     }
   }

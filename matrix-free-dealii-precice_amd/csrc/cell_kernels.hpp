@@ -359,20 +359,32 @@ struct QSolidT2F0 {
 };
 
 // Heat: value v, gradient c*dt*grad v (heat_transfer.h:242-248).
-// Fields: Jinv (9), JxW, c.
+// Fields: Jinv (9), JxW, c per point; or, when every cell is affine (the
+// affine-cell geometry of MatrixFree), Jinv and det J once per cell
+// (cell_geo, 10 words), JxW = det J * wq[point], and c the only field.
 template <typename Number>
 struct QHeat {
   static constexpr int NC = 1;
-  static constexpr int kFields = 11;
+  static constexpr int kFields = 11;  // 1 with cell_geo
   static constexpr bool kNodalField = false;
   const Number *qd;
   int64_t nqp;
   Number dt;
+  const Number *cell_geo;  // per stored cell: Jinv (9), det J; or null
+  const Number *wq;        // N^3 tensor-product quadrature weights (with cell_geo)
   struct Data {
     Number ji[9], jxw, c;
   };
   __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
+    if (cell_geo) {
+      const Number *g = cell_geo + (int64_t)cell * 10;
+#pragma unroll
+      for (int f = 0; f < 9; ++f) d.ji[f] = g[f];
+      d.jxw = g[9] * wq[pt];
+      d.c = stream_load(qd + q);
+      return d;
+    }
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
     d.jxw = stream_load(qd + 9 * nqp + q);

@@ -29,6 +29,12 @@ const Number *t2_cell_jxw(const OperatorData &op) {
   return op.affine_ref ? op.cell_jxw.as<Number>() : nullptr;
 }
 
+// per-cell geometry of the heat op on affine meshes, or null (per-point fields)
+template <typename Number>
+const Number *heat_cell_geo(const OperatorData &op) {
+  return op.affine_ref ? op.cell_geo.as<Number>() : nullptr;
+}
+
 template <typename Number>
 static void fill(Number *x, int64_t n, Number v, hipStream_t s) {
   if (n <= 0) return;
@@ -134,7 +140,7 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
         launch_simple(apply_cells<P, CPB, Number, QSolidT2F0<Number>>, q);
     }
   } else {
-    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt), heat_cell_geo<Number>(op), op.wq3.as<Number>()};
     if (!launch_bundles(q))
       launch_simple(apply_cells<P, CPB, Number, QHeat<Number>>, q);
   }
@@ -200,7 +206,7 @@ static void launch_diagonal(OperatorData &op) {
         diagonal_cells<P, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
       }
     } else {
-      QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt)};
+      QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt), heat_cell_geo<Number>(op), op.wq3.as<Number>()};
       diagonal_cells<P, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
     }
     MFG_HIP(hipGetLastError());
@@ -609,7 +615,7 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
   const Tables1D t = make_tables(p);
   if (precision == MFG_FP64) upload_tables<double>(*this, t);
   else upload_tables<float>(*this, t);
-  if (kind == 0 && affine_ref) {
+  if (affine_ref) {
     const int N = p + 1;
     std::vector<double> w3(N3);
     for (int x = 0; x < N3; ++x) w3[x] = t.weight[x % N] * t.weight[(x / N) % N] * t.weight[x / (N * N)];
@@ -629,8 +635,38 @@ static void heat_geometry(OperatorData &op, const mfg_level_desc &d, const doubl
   // per (cell, q): Jinv (9), JxW, c -- MappingQ1 data (heat_transfer.h:483-496)
   const Tables1D t = make_tables(op.p);
   const int N = op.p + 1, N3 = N * N * N;
-  std::vector<Number> h((size_t)11 * op.nqp);
   const int64_t n = op.nqp;
+  if (op.affine_ref) {
+    // affine cells: the geometry once per cell, the coefficient per point
+    std::vector<Number> geo((size_t)op.n_cells * 10), cq((size_t)n);
+    for (int c = 0; c < op.n_cells; ++c) {
+      const int64_t orig = op.perm[c];
+      const double *X = d.cell_vertices + orig * 24;
+      double J[3][3];
+      for (int e = 0; e < 3; ++e) {
+        J[e][0] = X[3 + e] - X[e];
+        J[e][1] = X[6 + e] - X[e];
+        J[e][2] = X[12 + e] - X[e];
+      }
+      const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      MFG_REQUIRE(det > 0, MFG_ERR_ARGUMENT, "inverted cell");
+      const double id = 1.0 / det;
+      const double R[9] = {(J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                           (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id, (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id,
+                           (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id,
+                           (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                           (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id};
+      for (int f = 0; f < 9; ++f) geo[(size_t)c * 10 + f] = (Number)R[f];
+      geo[(size_t)c * 10 + 9] = (Number)det;
+      for (int q = 0; q < N3; ++q) cq[(size_t)c * N3 + q] = (Number)(coef ? coef[orig * N3 + q] : 1.0);
+    }
+    upload(op.cell_geo, geo.data(), geo.size(), op.stream);
+    upload(op.qd, cq.data(), cq.size(), op.stream);
+    return;
+  }
+  std::vector<Number> h((size_t)11 * op.nqp);
   for (int c = 0; c < op.n_cells; ++c)
     for (int q = 0; q < N3; ++q) {
       const int64_t orig = op.perm[c];  // colour-major cell c is input cell orig
@@ -668,7 +704,7 @@ static void heat_geometry(OperatorData &op, const mfg_level_desc &d, const doubl
 void OperatorData::setup_heat(const mfg_level_desc &d, const double *coef) {
   if (precision == MFG_FP64) heat_geometry<double>(*this, d, coef);
   else heat_geometry<float>(*this, d, coef);
-  n_fields = 11;
+  n_fields = affine_ref ? 1 : QHeat<double>::kFields;
   ready = true;
 }
 
@@ -693,7 +729,7 @@ void OperatorData::setup_solid() {
 }
 
 size_t OperatorData::memory_consumption() const {
-  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes + cell_jxw.bytes + wq3.bytes;
+  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes + cell_jxw.bytes + wq3.bytes + cell_geo.bytes;
 }
 
 }  // namespace mfg

@@ -25,6 +25,7 @@ struct OperatorData {
   // JxW_ref = cell_jxw[stored cell] * wq3[point] (tensor2, formulation 1)
   bool affine_ref = false;
   DeviceBuffer cell_jxw, wq3;
+  DeviceBuffer cell_geo;  // heat on affine cells: Jinv (9), det J per stored cell
   DeviceBuffer flag;         // int error flag
   std::vector<int32_t> host_constrained;
   std::vector<int32_t> host_cell_nodes;  // input cell order (used to build transfers)

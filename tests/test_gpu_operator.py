@@ -193,6 +193,29 @@ def test_heat_vmult_and_diagonal(cuda, p):
     assert rel(y, oo.vmult(x)) < TOL64
     go.compute_diagonal()
     assert rel(go.diagonal(), oo.diagonal()) < TOL64
+    assert go.n_fields() == 1  # affine cells: geometry per cell, the coefficient per point
+
+
+@pytest.mark.parametrize("p,precision", [(2, mfgpu.FP64), (4, mfgpu.FP64), (3, mfgpu.FP32)])
+def test_heat_non_affine_cells(cuda, p, precision):
+    """Distorted vertices: the heat op keeps Jinv, JxW and c per point (11 fields)."""
+    print(f"non-affine heat p={p} precision={precision}", flush=True)
+    lo, hi, n = [0, 0, 0], [1.0, 0.8, 0.6], [3, 2, 2]
+    om = O.Mesh(lo, hi, n, p, 1, 63, 1)
+    v = distort(om.cell_vertices())
+    om.set_cell_vertices(v)
+    gm = DistortedMesh(mfgpu.BoxMesh(lo, hi, n, p, 1, 63, 1), v)
+    coef = np.random.default_rng(p).uniform(0.5, 2.0, om.n_cells * (p + 1) ** 3)
+    go = mfgpu.LaplaceOperator(gm, 0.1, coef, precision=precision)
+    oo = O.Heat(om, dt=0.1, coef=coef)
+    assert go.n_fields() == 11
+    x = O.fill_uniform(6, om.n_dofs)
+    y = torch.empty(om.n_dofs, dtype=go.dtype, device="cuda")
+    go.vmult(y, dev(x, go.dtype))
+    tol = TOL64 if precision == mfgpu.FP64 else TOL32
+    assert rel(y.double(), oo.vmult(x)) < tol
+    go.compute_diagonal()
+    assert rel(go.diagonal().double(), oo.diagonal()) < tol
 
 
 @pytest.mark.parametrize("p", [2, 4])

@@ -78,6 +78,16 @@ def test_distorted_mesh_sumfact_and_cachings(p):
         assert np.linalg.norm(y - ys[0]) / np.linalg.norm(ys[0]) < 1e-12
 
 
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_distorted_mesh_heat_sumfact_matches_dense(p):
+    m = distorted(O.Mesh([0, 0, 0], [1.0, 2.0, 0.5], [2, 1, 2], p, 1, face_mask=0))
+    coef = np.linspace(0.5, 2.0, m.n_cells * (p + 1) ** 3)
+    h = O.Heat(m, dt=0.1, coef=coef)
+    x = O.fill_uniform(3, m.n_dofs)
+    yb = h.vmult_bruteforce(x)
+    assert np.linalg.norm(h.vmult(x) - yb) < 1e-13 * np.linalg.norm(yb)
+
+
 @pytest.mark.parametrize("form", [0, 1])
 def test_tangent_is_symmetric(form):
     m = unit_mesh(2, 2, 3)
