@@ -65,7 +65,15 @@ def cpu_baseline(seconds_budget=20.0):
         op.vmult(x)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": round(m.n_dofs / t / 1e9, 4), "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement of the reference (the oracle), one OpenMP thread per core",
+            "cpu_model": model,
             "sample": f"oracle tensor2 FP64 vmult, [0,1]^3 32^3 Q4 cells ({m.n_dofs} DoFs, same per-cell work "
                       f"as the 128x32x32 beam), median of {len(times)} vmults, {threads} OpenMP threads"}
 
@@ -117,9 +125,30 @@ def mgcg_bench(mfgpu, torch, cells, p, level_precision, numbering, x_true_seed=2
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
     err = float(torch.linalg.norm(x - xt) / torch.linalg.norm(xt))
+    # operator applications per level in the same solve (counted in a second,
+    # untimed solve): sum of their canonical bytes over the solve time
+    all_ops = ops if fine64 is ops[-1] else ops + [fine64]
+    for op in all_ops:
+        op.enable_timing(True)
+        op.timing()
+    x2 = torch.zeros_like(xt)
+    mfgpu.cg_solve(fine64, b, x2, mfgpu.PRECOND_MG, mg, tol_rel=1e-8, max_iter=500)
+    counts, bytes_total = [], 0
+    for op, m in zip(all_ops, meshes + [meshes[-1]]):
+        _, k = op.timing()
+        op.enable_timing(False)
+        es = 8 if op.dtype == torch.float64 else 4
+        nqp = m.n_cells * (p + 1) ** 3
+        bytes_total += k * (es * 2 * m.n_dofs + 4 * nqp + 18 * es * nqp)
+        counts.append(k)
+    del x2
     return {"levels": n_levels, "iterations": info.iterations, "solve_s": round(t_solve, 4),
             "setup_s": round(t_setup, 3), "rel_residual": info.final_residual / float(torch.linalg.norm(b)),
-            "err_vs_x_true": err}, x
+            "err_vs_x_true": err, "vmults_per_level": counts,
+            "achieved_gbs_vmults": round(bytes_total / t_solve / 1e9, 1),
+            "achieved_note": "canonical bytes of the operator applications only (no vector, transfer or "
+                             "coarse-solver bytes) / solve time; vmults_per_level lists levels 0.. then the "
+                             "FP64 outer operator when the V-cycle is FP32"}, x
 
 
 def batched_instances(mfgpu, n_inst=48, seed_base=3, coarse_eig_iterations=100, n_range=(4, 24)):
@@ -331,8 +360,15 @@ def heat_bench(mfgpu, torch, numbering, n=(96, 96, 96), p=4, reps=30):
             op.vmult(dst, src)
         ms, k = op.timing()
         ms /= k
+        es = 8 if prec == mfgpu.FP64 else 4
+        nqp = mesh.n_cells * (p + 1) ** 3
+        canonical = es * 2 * mesh.n_dofs + 4 * nqp + 11 * es * nqp  # J^-1, JxW and c per point
+        layout = es * 2 * mesh.n_dofs + 4 * nqp + op.n_fields() * es * nqp + (10 * es * mesh.n_cells if op.n_fields() == 1 else 0)
         out[key] = {"ms": round(ms, 4), "gdofs": round(mesh.n_dofs / (ms * 1e-3) / 1e9, 3),
-                    "cache_words_per_qp": op.n_fields()}
+                    "cache_words_per_qp": op.n_fields(), "canonical_bytes": canonical,
+                    "canonical_gbs": round(canonical / (ms * 1e-3) / 1e9, 1),
+                    "canonical_frac": round(canonical / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "layout_bytes": layout}
         log(f"heat {key}: {ms:.3f} ms")
         del op, src, dst
         torch.cuda.empty_cache()
@@ -414,6 +450,7 @@ def main():
     kern_avg_ms = kern_ms / max(launches, 1)
     gdofs = n_dof / (ms_per_step * 1e-3) / 1e9
     B = canonical_bytes(n_dof, n_cell, p)
+    n_qp_all = n_cell * (p + 1) ** 3
     achieved = B / (kern_avg_ms * 1e-3) / 1e9
 
     # per-vmult latency distribution (events around each call)
@@ -447,7 +484,9 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p, args.numbering)),
                      "traffic_source": "profiles/vmult_q4_beam_fp64_pmc_traffic.md",
                      "kernel": "apply_cells<4,2,double,QSolidT2F1>", "kernel_avg_ms": round(kern_avg_ms, 4),
-                     "canonical_bytes_per_launch": B},
+                     "canonical_bytes_per_launch": B,
+                     "layout_bytes_per_launch": 8 * 2 * n_dof + 4 * n_qp_all + op.n_fields() * 8 * n_qp_all
+                     + (8 * n_cell if op.n_fields() == 16 else 0)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
     }
     del op, src, dst

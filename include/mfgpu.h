@@ -248,6 +248,18 @@ int mfg_mg_setup(mfg_mg mg);
 int mfg_mg_vmult(mfg_mg mg, void *dst, const void *src, int32_t vec_precision);
 /* Chebyshev data of a level: lmax, lmin estimates and polynomial degree. */
 int mfg_mg_level_info(mfg_mg mg, int32_t level, double *lmax, double *lmin, int32_t *degree);
+/* V-cycle launch mode (no counterpart in the reference, whose V-cycle is a
+ * host loop over levels).  enable != 0: from the second V-cycle on, the
+ * sweeps above the coarse level replay as two captured hipGraphs on a stream
+ * of the hierarchy's own, ordered after and before the fine operator's
+ * stream; the coarse solve runs eagerly between them.  Results are bitwise
+ * those of the eager V-cycle.  0 (the default): every V-cycle launches
+ * eagerly -- on MI355X the graphs are not faster (the small levels are bound
+ * by each colour launch's critical path) and re-capturing after every setup
+ * adds latency.
+ * graph_nodes (nullable) receives the node count of the captured graphs. */
+int mfg_mg_use_graph(mfg_mg mg, int32_t enable);
+int mfg_mg_graph_info(mfg_mg mg, int64_t *graph_nodes);
 
 /* ------------------------------------------------------------------- CG --- */
 /* SolverCG (Solid::solve_linear_system, mf_elasticity.h:1981-2070): solve

@@ -5,6 +5,7 @@
 // classes they configure (PreconditionChebyshev, SolverCG, Multigrid) follow
 // the library's documented behaviour.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 
@@ -405,6 +406,7 @@ void MGLevel::chebyshev(void *x, const void *b, bool zero_guess) {
 
 // ------------------------------------------------------------- the MG ---
 void MGData::setup() {
+  drop_graphs();  // new smoother coefficients
   for (auto &L : levels) {
     if (!L.op->has_diagonal) L.op->compute_diagonal();
     if (L.sol.bytes != (size_t)L.op->n_dofs * L.op->elem_size()) L.allocate();
@@ -459,24 +461,120 @@ void MGData::coarse_solve() {
   }
 }
 
-// Multigrid::level_v_step
-void MGData::v_cycle(int l) {
-  if (l == 0) {
-    coarse_solve();
-    return;
+void MGData::down_sweep() {
+  for (int l = (int)levels.size() - 1; l >= 1; --l) {
+    MGLevel &L = levels[l];
+    MGLevel &C = levels[l - 1];
+    const OperatorData &A = *L.op;
+    L.chebyshev(L.sol.ptr, L.def.ptr, true);  // pre-smoothing from a zero guess
+    op_vmult(A, L.tmp.ptr, L.sol.ptr);
+    vec_residual(A.precision, L.tmp.ptr, L.def.ptr, A.n_dofs, A.stream);
+    vec_fill(C.op->precision, C.def.ptr, C.op->n_dofs, 0.0, A.stream);
+    transfers[l - 1]->restrict_add(C.def.ptr, L.tmp.ptr);
   }
-  MGLevel &L = levels[l];
-  MGLevel &C = levels[l - 1];
-  const OperatorData &A = *L.op;
-  hipStream_t s = A.stream;
-  L.chebyshev(L.sol.ptr, L.def.ptr, true);  // pre-smoothing from a zero guess
-  op_vmult(A, L.tmp.ptr, L.sol.ptr);
-  vec_residual(A.precision, L.tmp.ptr, L.def.ptr, A.n_dofs, s);
-  vec_fill(C.op->precision, C.def.ptr, C.op->n_dofs, 0.0, s);
-  transfers[l - 1]->restrict_add(C.def.ptr, L.tmp.ptr);
-  v_cycle(l - 1);
-  transfers[l - 1]->prolongate_add(L.sol.ptr, C.sol.ptr);
-  L.chebyshev(L.sol.ptr, L.def.ptr, false);  // post-smoothing
+}
+
+void MGData::up_sweep() {
+  for (int l = 1; l < (int)levels.size(); ++l) {
+    MGLevel &L = levels[l];
+    MGLevel &C = levels[l - 1];
+    transfers[l - 1]->prolongate_add(L.sol.ptr, C.sol.ptr);
+    L.chebyshev(L.sol.ptr, L.def.ptr, false);  // post-smoothing
+  }
+}
+
+bool MGData::graphs_allowed() const {
+  if (!use_graph || levels.size() < 2) return false;
+  for (const MGLevel &L : levels)
+    if (L.op->timing) return false;  // the timing events belong to eager launches
+  return true;
+}
+
+void MGData::drop_graphs() {
+  if (graph_down) (void)hipGraphExecDestroy(graph_down);
+  if (graph_up) (void)hipGraphExecDestroy(graph_up);
+  graph_down = graph_up = nullptr;
+  graph_versions.clear();
+  graph_calls = 0;
+  graph_nodes = 0;
+}
+
+MGData::~MGData() {
+  drop_graphs();
+  if (graph_in) (void)hipEventDestroy(graph_in);
+  if (graph_out) (void)hipEventDestroy(graph_out);
+  if (graph_stream) (void)hipStreamDestroy(graph_stream);
+}
+
+static hipGraphExec_t capture_on(hipStream_t s, const std::function<void()> &body, size_t &nodes) {
+  hipGraph_t g = nullptr;
+  MFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  MFG_HIP(hipStreamEndCapture(s, &g));
+  size_t nn = 0;
+  (void)hipGraphGetNodes(g, nullptr, &nn);
+  nodes += nn;
+  hipGraphExec_t e = nullptr;
+  const hipError_t st = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MFG_HIP(st);
+  return e;
+}
+
+void MGData::run_graphed(hipStream_t caller) {
+  if (!graph_stream) {
+    MFG_HIP(hipStreamCreateWithFlags(&graph_stream, hipStreamNonBlocking));
+    MFG_HIP(hipEventCreateWithFlags(&graph_in, hipEventDisableTiming));
+    MFG_HIP(hipEventCreateWithFlags(&graph_out, hipEventDisableTiming));
+  }
+  std::vector<uint64_t> versions;
+  for (const MGLevel &L : levels) versions.push_back(L.op->buffers_version);
+  if (versions != graph_versions) {
+    if (graph_down) (void)hipGraphExecDestroy(graph_down);
+    if (graph_up) (void)hipGraphExecDestroy(graph_up);
+    graph_down = graph_up = nullptr;
+    graph_nodes = 0;
+    graph_versions = versions;
+  }
+  MFG_HIP(hipEventRecord(graph_in, caller));
+  MFG_HIP(hipStreamWaitEvent(graph_stream, graph_in, 0));
+  // every level operator (and with it each transfer, which launches on its
+  // coarse operator's stream) works on the graph stream meanwhile
+  std::vector<hipStream_t> saved;
+  for (MGLevel &L : levels) {
+    saved.push_back(L.op->stream);
+    L.op->stream = graph_stream;
+  }
+  auto restore = [&] {
+    for (size_t l = 0; l < levels.size(); ++l) levels[l].op->stream = saved[l];
+  };
+  try {
+    if (!graph_down) graph_down = capture_on(graph_stream, [&] { down_sweep(); }, graph_nodes);
+    MFG_HIP(hipGraphLaunch(graph_down, graph_stream));
+    coarse_solve();
+    if (!graph_up) graph_up = capture_on(graph_stream, [&] { up_sweep(); }, graph_nodes);
+    MFG_HIP(hipGraphLaunch(graph_up, graph_stream));
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
+  MFG_HIP(hipEventRecord(graph_out, graph_stream));
+  MFG_HIP(hipStreamWaitEvent(caller, graph_out, 0));
+}
+
+// Multigrid::level_v_step from the finest level: the down sweep, the coarse
+// solve and the up sweep (the same launches the graphs replay)
+void MGData::v_cycle() {
+  down_sweep();
+  coarse_solve();
+  up_sweep();
 }
 
 // PreconditionMG::vmult: copy_to_mg, V-cycle, copy_from_mg
@@ -484,9 +582,11 @@ void MGData::vmult(void *dst, const void *src, int vec_prec) {
   MFG_REQUIRE(ready, MFG_ERR_STATE, "mfg_mg_setup() not called");
   MGLevel &F = levels.back();
   const int64_t n = F.op->n_dofs;
-  vec_convert(F.op->precision, F.def.ptr, vec_prec, src, n, F.op->stream);
-  v_cycle((int)levels.size() - 1);
-  vec_convert(vec_prec, dst, F.op->precision, F.sol.ptr, n, F.op->stream);
+  hipStream_t caller = F.op->stream;
+  vec_convert(F.op->precision, F.def.ptr, vec_prec, src, n, caller);
+  if (graphs_allowed() && ++graph_calls >= 2) run_graphed(caller);
+  else v_cycle();
+  vec_convert(vec_prec, dst, F.op->precision, F.sol.ptr, n, caller);
 }
 
 // ------------------------------------------------------------------ CG ---
@@ -633,6 +733,21 @@ int mfg_mg_vmult(mfg_mg mg, void *dst, const void *src, int32_t vec_precision) {
   return guarded([&] {
     MFG_REQUIRE(mg && dst && src, MFG_ERR_ARGUMENT, "null argument");
     mg->vmult(dst, src, vec_precision);
+  });
+}
+
+int mfg_mg_use_graph(mfg_mg mg, int32_t enable) {
+  return guarded([&] {
+    MFG_REQUIRE(mg, MFG_ERR_ARGUMENT, "null argument");
+    mg->use_graph = enable != 0;
+    mg->drop_graphs();
+  });
+}
+
+int mfg_mg_graph_info(mfg_mg mg, int64_t *graph_nodes) {
+  return guarded([&] {
+    MFG_REQUIRE(mg && graph_nodes, MFG_ERR_ARGUMENT, "null argument");
+    *graph_nodes = (int64_t)mg->graph_nodes;
   });
 }
 

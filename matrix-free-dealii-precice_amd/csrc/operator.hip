@@ -291,8 +291,11 @@ OperatorData::~OperatorData() {
 void OperatorData::compute_diagonal() {
   MFG_REQUIRE(ready, MFG_ERR_STATE, "compute_diagonal before cache()");
   const size_t es = precision == MFG_FP64 ? 8 : 4;
-  if (diag.bytes != n_dofs * es) diag.allocate(n_dofs * es);
-  if (inv_diag.bytes != n_dofs * es) inv_diag.allocate(n_dofs * es);
+  if (diag.bytes != n_dofs * es || inv_diag.bytes != n_dofs * es) {
+    diag.allocate(n_dofs * es);
+    inv_diag.allocate(n_dofs * es);
+    ++buffers_version;
+  }
   fill<int>(flag.as<int>(), 1, 0, stream);
   const int threads = 256;
   if (precision == MFG_FP64) {
@@ -326,7 +329,10 @@ void OperatorData::cache(const void *u, int u_precision) {
   fill<int>(flag.as<int>(), 1, 0, stream);
   if (sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {  // keeps the displacement itself (mf_nh_operator.h:442-455)
     const int threads = 256, blocks = std::min(blocks_for(n_dofs, threads), 65536);
-    if (u_nodal.bytes != (size_t)n_dofs * elem_size()) u_nodal.allocate((size_t)n_dofs * elem_size());
+    if (u_nodal.bytes != (size_t)n_dofs * elem_size()) {
+      u_nodal.allocate((size_t)n_dofs * elem_size());
+      ++buffers_version;
+    }
     if (precision == MFG_FP64 && u_precision == MFG_FP64)
       convert_kernel<double, double><<<blocks, threads, 0, stream>>>(u_nodal.as<double>(), (const double *)u, n_dofs);
     else if (precision == MFG_FP64)
@@ -533,6 +539,7 @@ bool OperatorData::build_bundles(int width) {
   upload(bundle_cells, cells_tab.data(), cells_tab.size(), stream);
   upload(bundle_len, lens.data(), lens.size(), stream);
   upload(bundle_map, bmap.data(), bmap.size(), stream);
+  ++buffers_version;
   bundle_steps = S;
   bundle_width = width;
   return true;
@@ -922,6 +929,7 @@ int mfg_operator_set_cell_schedule(mfg_operator op, int32_t schedule) {
     op->cell_schedule = MFG_CELLS_COLOURS;
     if (schedule != MFG_CELLS_COLOURS && op->build_bundles(schedule == MFG_CELLS_BUNDLES ? 4 : 1))
       op->cell_schedule = schedule;
+    ++op->buffers_version;  // another kernel path: captured graphs are stale
   });
 }
 

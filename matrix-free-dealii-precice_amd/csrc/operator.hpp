@@ -42,6 +42,9 @@ struct OperatorData {
   std::vector<int> bundle_colour_start;
   DeviceBuffer bundle_cells, bundle_len, bundle_map;
   int cell_schedule = MFG_CELLS_COLOURS;
+  // bumped whenever a device buffer that apply() reads is reallocated (graphs
+  // that captured the old addresses must be re-captured)
+  uint64_t buffers_version = 0;
   bool build_bundles(int width);  // false if the mesh is not a structured box
   // optional cell-kernel timing
   bool timing = false;

@@ -63,8 +63,34 @@ struct MGData {
   int coarse_kind() const { return levels.size() == 1 ? 0 : prm.coarse_kind; }
   void setup();
   void vmult(void *dst, const void *src, int vec_prec);
-  void v_cycle(int l);
+  void v_cycle();
   void coarse_solve();
+  // Multigrid::level_v_step unrolled: pre-smoothing, residual and restriction
+  // from the finest level down to level 1, then (after the coarse solve)
+  // prolongation and post-smoothing back up.
+  void down_sweep();
+  void up_sweep();
+  // Optionally the two sweeps are captured into hipGraphs on the second
+  // V-cycle and replayed on a stream of their own; the coarse solve, which may
+  // synchronise with the host, runs eagerly between the two replays.  A graph
+  // is re-captured after setup() or when a level operator reallocated a
+  // buffer it reads.  Off by default: the small levels are bound by each
+  // colour launch's own critical path on the device, not by launch cost, and
+  // the re-capture after every setup() adds latency.
+  bool use_graph = false;
+  hipStream_t graph_stream = nullptr;
+  hipEvent_t graph_in = nullptr, graph_out = nullptr;
+  hipGraphExec_t graph_down = nullptr, graph_up = nullptr;
+  std::vector<uint64_t> graph_versions;
+  int graph_calls = 0;
+  size_t graph_nodes = 0;
+  bool graphs_allowed() const;
+  void run_graphed(hipStream_t caller);
+  void drop_graphs();
+  MGData() = default;
+  MGData(const MGData &) = delete;
+  MGData &operator=(const MGData &) = delete;
+  ~MGData();
 };
 
 struct CGWorkspace {

@@ -55,6 +55,8 @@ SIGNATURES = {
     "mfg_box_mesh_describe": (ctypes.c_int, [_vp, ctypes.POINTER(LevelDesc)]),
     "mfg_box_mesh_node_coords": (ctypes.c_int, [_vp, _c_dp]),
     "mfg_box_mesh_renumber": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_mg_use_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_mg_graph_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mfg_box_mesh_numbering": (ctypes.c_int32, [_vp]),
     "mfg_solid_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), ctypes.POINTER(SolidParams), ctypes.c_int32,
                                          ctypes.POINTER(_vp)]),
@@ -402,6 +404,15 @@ class Multigrid:
 
     def setup(self):
         check(lib().mfg_mg_setup(self.h))
+
+    def use_graph(self, enable=True):
+        """Replay the V-cycle's sweeps above the coarse level as captured hipGraphs (default off)."""
+        check(lib().mfg_mg_use_graph(self.h, 1 if enable else 0))
+
+    def graph_nodes(self):
+        n = ctypes.c_int64(0)
+        check(lib().mfg_mg_graph_info(self.h, ctypes.byref(n)))
+        return n.value
 
     def vmult(self, dst, src):
         import torch

@@ -165,3 +165,41 @@ def test_mixed_precision_mg_cg(cuda):
     print(f"mixed: iterations {info.iterations} (fp64 oracle {it_o}), err vs fp64 solve {rel(x, xo):.2e}")
     assert info.iterations <= it_o + 3
     assert rel(x, xo) < 1e-6
+
+
+@pytest.mark.parametrize("coarse_kind,precisions", [(1, None), (0, None), (1, [mfgpu.FP64, mfgpu.FP32, mfgpu.FP32])])
+def test_graphed_vcycle_equals_eager(cuda, coarse_kind, precisions):
+    """The V-cycle's sweeps replayed as hipGraphs give bitwise the eager
+    results: over several replays with eager work in between, after a new
+    linearisation (re-setup and re-capture), and through MG-CG."""
+    print(f"graphed V-cycle coarse_kind={coarse_kind} precisions={precisions}", flush=True)
+    gms, oms, gops, oops, trs = hierarchy("solid", [0, 0, 0], [2.0, 1.0, 1.0], [2, 1, 1], 3, 3, precisions=precisions)
+    eager = mfgpu.Multigrid(gops, trs, cheb_degree=5, coarse_kind=coarse_kind)
+    graphed = mfgpu.Multigrid(gops, trs, cheb_degree=5, coarse_kind=coarse_kind)
+    eager.use_graph(False)
+    graphed.use_graph(True)
+    n = gms[-1].n_dofs
+    for relinearised in (False, True):
+        if relinearised:
+            for op, m in zip(gops, gms):  # another linearisation point on every level
+                op.cache(dev(m.u0(amp=2e-3, noise=1e-5, seed=9), op.dtype))
+        eager.setup()
+        graphed.setup()
+        for rep in range(4):
+            src = mfgpu.fill_uniform(20 + rep, torch.empty(n, dtype=torch.float64, device="cuda"))
+            want = torch.empty_like(src)
+            got = torch.full_like(src, float("nan"))
+            eager.vmult(want, src)
+            graphed.vmult(got, src)
+            assert torch.equal(got, want), (relinearised, rep)
+            torch.cuda.synchronize()
+            _ = float(src.sum())  # eager work between replays
+        assert graphed.graph_nodes() > 0 and eager.graph_nodes() == 0
+    fine = gops[-1] if precisions is None else mfgpu.NeoHookOperator(gms[-1], O.MU, O.NU, O.RHO, O.ALPHA1)
+    if precisions is not None:
+        fine.cache(dev(gms[-1].u0(amp=2e-3, noise=1e-5, seed=9)))
+    b = mfgpu.fill_uniform(30, torch.empty(n, dtype=torch.float64, device="cuda"))
+    x1, x2 = torch.zeros_like(b), torch.zeros_like(b)
+    i1 = mfgpu.cg_solve(fine, b, x1, mfgpu.PRECOND_MG, eager, tol_rel=1e-8)
+    i2 = mfgpu.cg_solve(fine, b, x2, mfgpu.PRECOND_MG, graphed, tol_rel=1e-8)
+    assert i1.iterations == i2.iterations and torch.equal(x1, x2)
