@@ -164,10 +164,10 @@ def batched_instances(mfgpu, n_inst=48, seed_base=3, coarse_eig_iterations=100, 
         p = int(r.integers(1, 7))
         n = [int(x) for x in r.integers(n_range[0], n_range[1] + 1, size=3)]
         c = float(r.uniform(0.5, 2.0))
-        # colour cell schedule: the batched kernels' cell order, so batched and sequential results are bitwise equal
+        # each instance on its default cell schedule (colours, or the gather for small operators): both sum a
+        # node's cell contributions in colour order, as the batched kernels do, so results are bitwise equal
         out.append(mfgpu.Instance(kind, n, p, fine_precision=mfgpu.FP64, level_precision=mfgpu.FP32, dt=0.1,
-                                  conductivity=c, coarse_eig_iterations=coarse_eig_iterations, seed=seed_base + k,
-                                  cell_schedule=mfgpu.CELLS_COLOURS))
+                                  conductivity=c, coarse_eig_iterations=coarse_eig_iterations, seed=seed_base + k))
     return out
 
 

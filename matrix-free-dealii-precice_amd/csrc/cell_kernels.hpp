@@ -628,8 +628,10 @@ struct ApplyArgs {
 // the workgroup, thread t owning point t % N3 of local cell t / N3; threads
 // past cpb*N3 only join the barriers.  `assign`: the first colour touching a
 // node stores instead of adding.  Shape and collocation tables must be in LDS
-// before the first barrier inside.
-template <int P, typename Number, class QOp>
+// before the first barrier inside.  kToCells: instead of the read-modify-write
+// of dst, each point stores its node's contribution (NC values) at
+// dst[(cell N3 + point) NC + c], for the gather of the GATHER schedule.
+template <int P, typename Number, class QOp, bool kToCells = false>
 __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ cell_nodes,
                                   const Number *__restrict__ src, Number *__restrict__ dst, int cell_begin,
                                   int cell_end, int cpb, int assign, const Number *s_shape, const Number *s_colloc,
@@ -669,7 +671,7 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   constexpr bool kLateDst = sizeof(Number) == 8 && P <= 4;
   const bool first = assign && (entry >> 31);
   Number dold[NC];
-  if constexpr (!kLateDst) {
+  if constexpr (!kLateDst && !kToCells) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const bool skip = !active || first || ((entry >> (kConstraintShift + c)) & 1u);
@@ -693,6 +695,13 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   qop.apply(qd, val, grad);
   Number out[NC];
   integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
+  if constexpr (kToCells) {
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dst[qidx * NC + c] = out[c];
+    }
+    return;
+  }
   if constexpr (kLateDst) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -730,6 +739,59 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
   const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
   cell_group<P, Number, QOp>(qop, a.cell_nodes, a.src, a.dst, first_cell, a.cell_end, CPB, assign, s_shape,
                              s_colloc, s_tile);
+}
+
+// GATHER schedule, first launch: every cell at once, each point storing its
+// contribution per cell (no colours, no read of dst).
+template <int P, int CPB, typename Number, class QOp>
+__global__ void __launch_bounds__((block_threads<P, CPB>()))
+    apply_cells_to_cells(ApplyArgs<Number> a, QOp qop, Number *contrib) {
+  constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC;
+  __shared__ Number s_shape[N * N], s_colloc[N * N];
+  __shared__ Number s_tile[CPB * 3 * NC * N3];
+  const int t = threadIdx.x;
+  if (t < N * N) {
+    s_shape[t] = a.shape[t];
+    s_colloc[t] = a.colloc[t];
+  }
+  const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
+  cell_group<P, Number, QOp, true>(qop, a.cell_nodes, a.src, contrib, first_cell, a.cell_end, CPB, 1, s_shape,
+                                   s_colloc, s_tile);
+}
+
+// GATHER schedule, second launch: one thread per node sums the contributions
+// of its cells in colour order -- the order in which the colour launches add
+// them, so the result is theirs bit for bit (up to the sign of a zero).
+// fix_mode 1 (vmult): constrained components copy src, the others are
+// overwritten; 2 (vmult_add): constrained components add src, the others add
+// to dst.
+template <typename Number, int NC>
+__global__ void gather_contributions(const Number *__restrict__ contrib, const int32_t *__restrict__ offsets,
+                                     const int32_t *__restrict__ points, const uint8_t *__restrict__ cmask,
+                                     const Number *__restrict__ src, Number *__restrict__ dst, int64_t n_nodes,
+                                     int fix_mode) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= n_nodes) return;
+  const int b = offsets[node], e = offsets[node + 1];
+  const unsigned m = cmask[node];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int64_t dof = node * NC + c;
+    if ((m >> c) & 1u) {
+      dst[dof] = fix_mode == 1 ? src[dof] : dst[dof] + src[dof];
+      continue;
+    }
+    int k = b;
+    Number acc;
+    if (fix_mode == 1) {
+      acc = contrib[(int64_t)points[k] * NC + c];
+      ++k;
+    } else {
+      acc = dst[dof];
+    }
+    for (; k < e; ++k) acc = acc + contrib[(int64_t)points[k] * NC + c];
+    dst[dof] = acc;
+  }
 }
 
 // ------------------------------------------------------ bundle kernel ---

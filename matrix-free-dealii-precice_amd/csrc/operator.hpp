@@ -46,6 +46,11 @@ struct OperatorData {
   // that captured the old addresses must be re-captured)
   uint64_t buffers_version = 0;
   bool build_bundles(int width);  // false if the mesh is not a structured box
+  // GATHER schedule: per node, the points (stored cell * N3 + point) that
+  // touch it in colour order (CSR), its constrained components, and the
+  // per-point contribution buffer
+  DeviceBuffer gather_offsets, gather_points, gather_cmask, contrib;
+  void build_gather();
   // optional cell-kernel timing
   bool timing = false;
   mutable std::vector<std::pair<hipEvent_t, hipEvent_t>> events;

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libmfgpu.so")
 
 FP64, FP32 = 0, 1
 CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
-CELLS_COLOURS, CELLS_BUNDLES, CELLS_PENCILS = 0, 1, 2
+CELLS_COLOURS, CELLS_BUNDLES, CELLS_PENCILS, CELLS_GATHER = 0, 1, 2, 3
 NUMBERING = {"lexicographic": 0, "cellwise": 1}
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)

@@ -331,6 +331,43 @@ def test_cell_schedules(cuda, p, n):
         assert rel(yh.double(), href) < TOL32
 
 
+@pytest.mark.parametrize("kind,p,precision,caching,form", [
+    ("solid", 2, mfgpu.FP64, "tensor2", 1), ("solid", 4, mfgpu.FP64, "tensor2", 1), ("solid", 3, mfgpu.FP32, "tensor2", 1),
+    ("solid", 2, mfgpu.FP64, "tensor2", 0), ("solid", 3, mfgpu.FP64, "scalar_referential", 1),
+    ("solid", 2, mfgpu.FP64, "tensor4", 1), ("heat", 3, mfgpu.FP64, None, 1), ("heat", 5, mfgpu.FP32, None, 1)])
+def test_gather_schedule_equals_colours(cuda, kind, p, precision, caching, form):
+    """GATHER sums each node's cell contributions in colour order: vmult and
+    vmult_add are bitwise the colour schedule's, Dirichlet rows included."""
+    print(f"gather schedule {kind} p={p} precision={precision} caching={caching} formulation={form}", flush=True)
+    n = [3, 2, 3]
+    if kind == "solid":
+        m = mfgpu.BoxMesh([0, 0, 0], [1.0, 0.6, 0.4], n, p, 3, 1 | 8, 7, numbering="cellwise")
+        op = mfgpu.NeoHookOperator(m, O.MU, O.NU, O.RHO, O.ALPHA1, formulation=form, caching=caching,
+                                   precision=precision)
+        op.cache(torch.from_numpy(m.u0(amp=1e-2, noise=1e-5)).to("cuda", op.dtype))
+    else:
+        m = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], n, p, 1, 63, 1)
+        coef = np.random.default_rng(p).uniform(0.5, 2.0, m.n_cells * (p + 1) ** 3)
+        op = mfgpu.LaplaceOperator(m, 0.1, coef, precision=precision)
+    x = mfgpu.fill_uniform(7, torch.empty(m.n_dofs, dtype=op.dtype, device="cuda"))
+    z0 = mfgpu.fill_uniform(8, torch.empty_like(x))
+    res = {}
+    for sched in (mfgpu.CELLS_COLOURS, mfgpu.CELLS_GATHER, mfgpu.CELLS_COLOURS):
+        op.set_cell_schedule(sched)
+        assert op.cell_schedule() == sched
+        y = torch.full_like(x, float("nan"))
+        op.vmult(y, x)
+        z = z0.clone()
+        op.vmult_add(z, x)
+        if sched in res:
+            assert torch.equal(y, res[sched][0]) and torch.equal(z, res[sched][1])
+        res[sched] = (y, z)
+    assert torch.equal(res[mfgpu.CELLS_GATHER][0], res[mfgpu.CELLS_COLOURS][0])
+    assert torch.equal(res[mfgpu.CELLS_GATHER][1], res[mfgpu.CELLS_COLOURS][1])
+    con = torch.from_numpy(m.constrained().astype(np.int64)).cuda()
+    assert torch.equal(res[mfgpu.CELLS_GATHER][0][con], x[con])
+
+
 def test_error_paths(cuda):
     gm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
     go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
