@@ -174,10 +174,14 @@ size_t estimate_solid_problem_bytes(int n_levels, const mfg_level_desc *levels, 
     const int64_t nqp = (int64_t)d.n_cells * n3, ndof = d.n_nodes * d.n_components;
     const size_t es = (l == 0 || level_precision == MFG_FP64) ? 8 : 4;
     b += nqp * (18 * es + 4) + (size_t)d.n_cells * 24 * 8 + 11 * ndof * es;  // caches, map, vertices, diag + MG vectors
+    if (nqp <= kGatherMaxPoints) b += gather_bytes(nqp, d.n_nodes, d.n_components, es);
     if (l > 0) b += (size_t)levels[l - 1].n_cells * (2 * d.degree + 1) * (2 * d.degree + 1) * (2 * d.degree + 1) * 4;
     if (l == n_levels - 1) {
       b += 12 * ndof * 8;  // state, residual, CG workspace
-      if (level_precision == MFG_FP32 && n_levels > 1) b += nqp * (18 * 8 + 4) + (size_t)d.n_cells * 24 * 8;
+      if (level_precision == MFG_FP32 && n_levels > 1) {
+        b += nqp * (18 * 8 + 4) + (size_t)d.n_cells * 24 * 8;
+        if (nqp <= kGatherMaxPoints) b += gather_bytes(nqp, d.n_nodes, d.n_components, 8);
+      }
     }
   }
   return b;

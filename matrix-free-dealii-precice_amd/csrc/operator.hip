@@ -562,10 +562,6 @@ bool OperatorData::build_bundles(int width) {
   return true;
 }
 
-// operators with at most this many quadrature points use the GATHER schedule
-// unless another is set
-constexpr int64_t kGatherMaxPoints = 1 << 20;
-
 void OperatorData::build_gather() {
   const int N3 = (p + 1) * (p + 1) * (p + 1);
   std::vector<int32_t> offsets((size_t)n_nodes + 1, 0), points((size_t)nqp);

@@ -4,6 +4,15 @@
 
 namespace mfg {
 
+// operators with at most this many quadrature points use the GATHER cell
+// schedule unless another is set
+constexpr int64_t kGatherMaxPoints = 1 << 20;
+
+// device bytes of the GATHER schedule's tables and contribution buffer
+inline size_t gather_bytes(int64_t nqp, int64_t n_nodes, int nc, size_t es) {
+  return (size_t)nqp * (nc * es + 4) + (size_t)(n_nodes + 1) * 4 + (size_t)n_nodes;
+}
+
 struct OperatorData {
   int kind = 0;  // 0 solid (NeoHookOperator), 1 heat (LaplaceOperator)
   int precision = MFG_FP64;
