@@ -128,6 +128,27 @@ int mfg_solid_create(const mfg_level_desc *level, const mfg_solid_params *params
  * a host table per (cell, q) or NULL for c == 1 (Coefficient, :52-78). */
 int mfg_heat_create(const mfg_level_desc *level, double dt, const double *coefficient, int32_t precision,
                     mfg_operator *out);
+/* Physical coordinates of an operator's quadrature points (any kind):
+ * n_cells * (p+1)^3 * 3 doubles into device memory, input cell order, point
+ * i + N (j + N k) -- where the reference evaluates source terms and exact
+ * solutions (FEEvaluation::quadrature_point, heat_transfer.h:610-611, 836). */
+int mfg_operator_quadrature_points(mfg_operator op, double *xyz);
+/* LaplaceProblem::assemble_rhs (heat_transfer.h:577-655) without the preCICE
+ * interface flux: rhs = int (u_old + dt f) phi dx - A u_D, where A is the
+ * heat operator without constraints (the reference's inhomogeneous_operator)
+ * and u_D holds the Dirichlet values on the constrained dofs and zero
+ * elsewhere (constraints.distribute of a zero vector, :587-588).  u_old and
+ * u_D are read as stored.  f: the source at the quadrature points (device,
+ * layout of mfg_operator_quadrature_points without the 3; NULL = 0).  The
+ * constrained rows of rhs are set to zero (the solve's homogeneous
+ * constraints).  FP64 operator; device vectors; rhs must not alias an input. */
+int mfg_heat_assemble_rhs(mfg_operator op, double *rhs, const double *u_old, const double *u_dirichlet,
+                          const double *f_qp);
+/* LaplaceProblem::compute_error (heat_transfer.h:815-848):
+ * sqrt(sum over cells and points of (exact - u_h)^2 JxW), u_h read as stored,
+ * exact at the quadrature points (device, layout as f_qp).  Summed in a fixed
+ * order.  FP64 operator. */
+int mfg_heat_l2_error(mfg_operator op, const double *u, const double *exact_qp, double *error);
 int mfg_operator_destroy(mfg_operator op);                         /* clear() :112-113 */
 int mfg_operator_set_stream(mfg_operator op, void *hip_stream);
 int64_t mfg_operator_m(mfg_operator op);                             /* m() :133 */

@@ -56,6 +56,9 @@ SIGNATURES = {
     "mfg_box_mesh_node_coords": (ctypes.c_int, [_vp, _c_dp]),
     "mfg_box_mesh_renumber": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mfg_mg_use_graph": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_operator_quadrature_points": (ctypes.c_int, [_vp, _vp]),
+    "mfg_heat_assemble_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mfg_heat_l2_error": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
     "mfg_mg_graph_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "mfg_box_mesh_numbering": (ctypes.c_int32, [_vp]),
     "mfg_solid_create": (ctypes.c_int, [ctypes.POINTER(LevelDesc), ctypes.POINTER(SolidParams), ctypes.c_int32,
@@ -281,6 +284,13 @@ class _Operator:
         check(lib().mfg_operator_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def quadrature_points(self):
+        """Physical coordinates of the quadrature points, (n_cells * (p+1)^3, 3), input cell order."""
+        import torch
+        out = torch.empty(self.n_qp * 3, dtype=torch.float64, device="cuda")
+        check(lib().mfg_operator_quadrature_points(self.h, _ptr(out)))
+        return out.view(self.n_qp, 3)
+
     def set_cell_schedule(self, schedule):
         """CELLS_COLOURS or CELLS_BUNDLES (structured boxes; falls back to colours elsewhere)."""
         check(lib().mfg_operator_set_cell_schedule(self.h, schedule))
@@ -331,6 +341,17 @@ class LaplaceOperator(_Operator):
                                     ctypes.byref(h)))
         super().__init__(h, precision, mesh)
         self.n_qp = mesh.n_cells * (mesh.degree + 1) ** 3
+
+    def assemble_rhs(self, rhs, u_old, u_dirichlet, f_qp=None):
+        """LaplaceProblem::assemble_rhs: rhs = int (u_old + dt f) phi - A_unconstrained u_D, constrained rows 0."""
+        check(lib().mfg_heat_assemble_rhs(self.h, _ptr(rhs), _ptr(u_old), _ptr(u_dirichlet),
+                                          None if f_qp is None else _ptr(f_qp)))
+
+    def l2_error(self, u, exact_qp):
+        """LaplaceProblem::compute_error: sqrt(int (exact - u_h)^2)."""
+        e = ctypes.c_double(0.0)
+        check(lib().mfg_heat_l2_error(self.h, _ptr(u), _ptr(exact_qp), ctypes.byref(e)))
+        return e.value
 
 
 # ------------------------------------------------------------ multigrid ---
