@@ -563,6 +563,7 @@ bool OperatorData::build_bundles(int width) {
 }
 
 void OperatorData::build_gather() {
+  MFG_REQUIRE(nqp < ((int64_t)1 << 31), MFG_ERR_ARGUMENT, "too many quadrature points for the GATHER tables");
   const int N3 = (p + 1) * (p + 1) * (p + 1);
   std::vector<int32_t> offsets((size_t)n_nodes + 1, 0), points((size_t)nqp);
   for (int64_t x = 0; x < nqp; ++x) offsets[host_cell_nodes[x] + 1]++;
