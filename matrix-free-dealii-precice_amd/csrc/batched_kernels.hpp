@@ -256,13 +256,13 @@ __global__ void __launch_bounds__(kSegThreads)
       y[i] = v;
     } else if (OP == kSegChebFirstStep) {
       const T *b = (const T *)sg.b, *tt = (const T *)sg.c, *invD = (const T *)sg.d;
-      const T v = (T)((double)invD[i] * ((double)b[i] - (double)tt[i]) * sg.s0);
+      const T v = cheb_first_step_value<T>(invD[i], b[i], tt[i], sg.s0);
       ((T *)sg.a)[i] = v;
       y[i] += v;
     } else if (OP == kSegChebUpdate) {
       T *d = (T *)sg.a;
       const T *b = (const T *)sg.b, *tt = (const T *)sg.c, *invD = (const T *)sg.d;
-      const T v = (T)(sg.s0 * (double)d[i] + sg.s1 * (double)invD[i] * ((double)b[i] - (double)tt[i]));
+      const T v = cheb_update_value<T>(d[i], invD[i], b[i], tt[i], sg.s0, sg.s1);
       d[i] = v;
       y[i] += v;
     } else if (OP == kSegAddScaled) {

@@ -514,7 +514,7 @@ bool OperatorData::build_bundles(int width) {
   for (int64_t sc = 0; sc < n_cells; ++sc)
     for (int pt = 0; pt < N3; ++pt) {
       const int32_t node = cn[(int64_t)perm[sc] * N3 + pt];
-      bmap[(size_t)sc * N3 + pt] = (uint32_t)node | ((uint32_t)host_cmask[node] << kNodeBits);
+      bmap[(size_t)sc * N3 + pt] = (uint32_t)node | ((uint32_t)host_cmask[node] << kConstraintShift);
     }
   std::vector<char> touched((size_t)n_nodes, 0);
   bundle_colour_start.assign(1, 0);
@@ -621,7 +621,7 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
       // colour that writes this node stores instead of adding
       const uint32_t first = touched[node] ? 0u : (1u << 31);
       touched[node] = 1;
-      map[(size_t)c * N3 + x] = (uint32_t)node | ((uint32_t)cmask[node] << kNodeBits) | first;
+      map[(size_t)c * N3 + x] = (uint32_t)node | ((uint32_t)cmask[node] << kConstraintShift) | first;
     }
     for (int x = 0; x < 24; ++x) vert[(size_t)c * 24 + x] = d.cell_vertices[o * 24 + x];
   }

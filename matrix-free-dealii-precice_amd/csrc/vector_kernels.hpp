@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include "smoother_terms.hpp"
 
 namespace mfg {
 
@@ -132,7 +133,7 @@ template <typename T>
 __global__ void cheb_first_step_kernel(T *x, T *d, const T *b, const T *t, const T *invD, double inv_theta,
                                        int64_t n) {
   MFG_GRID_STRIDE(i, n) {
-    const T v = (T)((double)invD[i] * ((double)b[i] - (double)t[i]) * inv_theta);
+    const T v = cheb_first_step_value<T>(invD[i], b[i], t[i], inv_theta);
     d[i] = v;
     x[i] += v;
   }
@@ -142,7 +143,7 @@ template <typename T>
 __global__ void cheb_update_kernel(T *x, T *d, const T *b, const T *t, const T *invD, double f1, double f2,
                                    int64_t n) {
   MFG_GRID_STRIDE(i, n) {
-    const T v = (T)(f1 * (double)d[i] + f2 * (double)invD[i] * ((double)b[i] - (double)t[i]));
+    const T v = cheb_update_value<T>(d[i], invD[i], b[i], t[i], f1, f2);
     d[i] = v;
     x[i] += v;
   }
