@@ -338,6 +338,49 @@ def cachings_bench(mfgpu, torch, cells, p, numbering, reps=20):
     return out
 
 
+def newton_bench(mfgpu, torch, cells, p, numbering):
+    """One Newmark time step (Solid::solve_nonlinear_timestep: Newton with the
+    per-iteration cache, diagonal and eigenvalue set-up, MG-CG to 1e-8 per
+    linear solve) on the beam hierarchy from u0, body force (0, 0, -9.81):
+    FP64 levels, then FP32 levels with FP64 outer CG, residual, Newton update
+    and coarse level."""
+    n_levels = 1
+    while (cells[0] >> n_levels) >= 4 and (cells[0] >> n_levels) << n_levels == cells[0] and \
+            (cells[1] >> n_levels) >= 1 and (cells[2] >> n_levels) >= 1:
+        n_levels += 1
+    coarse = [c >> (n_levels - 1) for c in cells]
+    out = {"workload": f"one Newmark step (beta 0.25, dt {DT}) from u0, body force (0, 0, -9.81), Newton to "
+                       "||R||/||R0|| <= 1e-8 (at least 3 iterations), MG-CG to 1e-8, Chebyshev degree 5, "
+                       "coarse FP64 CG to 1e-3", "levels": n_levels}
+    us = {}
+    for lp, key in ((mfgpu.FP64, "fp64"), (mfgpu.FP32, "mixed_fp32_levels")):
+        meshes = [mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], [c << l for c in coarse], p, 3, 1, 7, numbering=numbering)
+                  for l in range(n_levels)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prob = mfgpu.SolidProblem(meshes, MU, NU, RHO, beta=BETA, dt=DT, body=(0.0, 0.0, -9.81), level_precision=lp,
+                                  tol_f=1e-8, tol_lin=1e-8, coarse_kind=1, coarse_tolerance=1e-3)
+        prob.set_state(u=torch.from_numpy(meshes[-1].u0(amp=1e-3, noise=1e-5, seed=0)).cuda())
+        torch.cuda.synchronize()
+        t_create = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        info = prob.step()
+        torch.cuda.synchronize()
+        t_step = time.perf_counter() - t0
+        u, _, _ = prob.get_state()
+        us[key] = u.clone()
+        out[key] = {"newton_iterations": info.newton_iterations, "linear_iterations": info.linear_iterations,
+                    "converged": bool(info.converged), "step_s": round(t_step, 4), "create_s": round(t_create, 3),
+                    "residual_reduction": info.residual_final / info.residual_0 if info.residual_0 else 0.0}
+        log(f"newton {key}: {info.newton_iterations} Newton, {info.linear_iterations} CG, {t_step:.3f} s")
+        del prob, u
+        torch.cuda.empty_cache()
+    out["u_diff_mixed_vs_fp64"] = float(torch.linalg.norm(us["mixed_fp32_levels"] - us["fp64"]) /
+                                        torch.linalg.norm(us["fp64"]))
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["step_s"] / out["mixed_fp32_levels"]["step_s"], 3)
+    return out
+
+
 def heat_bench(mfgpu, torch, numbering, n=(96, 96, 96), p=4, reps=30):
     """The heat operator M + dt c K (LaplaceOperator) on the unit cube, all
     faces Dirichlet, variable coefficient c ~ U[0.5, 2] per point: vmult time
@@ -386,6 +429,7 @@ def main():
     ap.add_argument("--only-batched", action="store_true", help="run the batched-instances leg alone")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-cachings", action="store_true", help="skip the MF-caching comparison leg")
+    ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-step leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: 2 vmults, then 2 copies of a 1 GiB buffer (known bytes), no timing")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
@@ -500,6 +544,9 @@ def main():
         out["mgcg"] = {"workload": "MG-CG to ||r||/||b|| <= 1e-8, rhs = A x_true (seed 2), Chebyshev degree 5, "
                                    "coarse FP64 CG to 1e-3", "fp64": r64, "mixed_fp32_vcycle": r32,
                        "speedup_mixed_vs_fp64": round(r64["solve_s"] / r32["solve_s"], 3)}
+    if not args.no_newton:
+        torch.cuda.empty_cache()
+        out["newton"] = newton_bench(mfgpu, torch, args.cells, p, args.numbering)
     if not args.no_cachings:
         torch.cuda.empty_cache()
         out["cachings"] = cachings_bench(mfgpu, torch, args.cells, p, args.numbering)
