@@ -30,7 +30,9 @@ NUMBERING = "cellwise"
 # HBM bytes per vmult of the current cell kernel from two rocprofv3 --pmc passes (FETCH_SIZE x 2.00 and
 # WRITE_SIZE x 1.00, factors from a copy of known size); PMC counters cannot be read from inside this
 # process, so the figure is re-measured whenever the kernel changes.
-MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise"): 3554406400, ((128, 32, 32), 4, "lexicographic"): 4017419059}
+MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise", "gradient"): 2709503181,
+                    ((128, 32, 32), 4, "cellwise", "folded"): 3554406400,
+                    ((128, 32, 32), 4, "lexicographic", "folded"): 4017419059}
 
 
 def log(msg):
@@ -315,8 +317,10 @@ def cachings_bench(mfgpu, torch, cells, p, numbering, reps=20):
     u0 = torch.from_numpy(mesh.u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
     src = mfgpu.fill_uniform(1, torch.empty(mesh.n_dofs, dtype=torch.float64, device=dev))
     out, ref = {}, None
-    for name in ("scalar_referential", "tensor2", "tensor4", "tensor4_ns"):
-        op = mfgpu.NeoHookOperator(mesh, MU, NU, RHO, ALPHA1, caching=name)
+    for name in ("scalar_referential", "tensor2", "tensor2_gradient", "tensor4", "tensor4_ns"):
+        op = mfgpu.NeoHookOperator(mesh, MU, NU, RHO, ALPHA1, caching=name.replace("_gradient", ""))
+        if name == "tensor2_gradient":
+            op.set_point_storage(mfgpu.T2_GRADIENT)
         op.cache(u0)
         dst = torch.empty_like(src)
         for _ in range(3):
@@ -437,6 +441,8 @@ def main():
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
     ap.add_argument("--numbering", default=NUMBERING, choices=["cellwise", "lexicographic"],
                     help="node numbering of the bench meshes")
+    ap.add_argument("--point-storage", default="gradient", choices=["folded", "gradient"],
+                    help="tensor2 point storage of the headline operator (16 or 10 words per point)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
@@ -460,6 +466,8 @@ def main():
     mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], args.cells, p, 3, 1, 7, numbering=args.numbering)
     n_dof, n_cell = mesh.n_dofs, mesh.n_cells
     op = mfgpu.NeoHookOperator(mesh, MU, NU, RHO, ALPHA1, formulation=1, caching="tensor2")
+    gradient_storage = args.point_storage == "gradient"
+    op.set_point_storage(mfgpu.T2_GRADIENT if gradient_storage else mfgpu.T2_FOLDED)
     u0 = torch.from_numpy(mesh.u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
     op.cache(u0)
     del u0
@@ -522,15 +530,18 @@ def main():
         "config": {"workload": f"tangent vmult, beam [0,4]x[0,1]^2, {args.cells[0]}x{args.cells[1]}x{args.cells[2]} "
                                f"cells, FE_Q({p})^3, QGauss({p + 1}), {n_dof} DoFs, u=0 on x=0, "
                                f"linearised at 1e-3 sin sin sin + 1e-5 noise, {args.numbering} node numbering",
-                   "n_dofs": n_dof, "n_cells": n_cell, "degree": p, "caching": "tensor2", "formulation": 1},
+                   "n_dofs": n_dof, "n_cells": n_cell, "degree": p, "caching": "tensor2", "formulation": 1,
+                   "point_storage": args.point_storage, "words_per_point": op.n_fields()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p, args.numbering)),
+                     "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p, args.numbering, args.point_storage)),
                      "traffic_source": "profiles/vmult_q4_beam_fp64_pmc_traffic.md",
-                     "kernel": "apply_cells<4,1,double,QSolidT2F1>", "kernel_avg_ms": round(kern_avg_ms, 4),
+                     "kernel": f"apply_cells<{p},1,double,{'QSolidT2F1G' if gradient_storage else 'QSolidT2F1'}>",
+                     "kernel_avg_ms": round(kern_avg_ms, 4),
                      "canonical_bytes_per_launch": B,
+                     # src + dst, map, the stored words per point, per-cell geometry (det J_ref; J_ref, J_ref^-1)
                      "layout_bytes_per_launch": 8 * 2 * n_dof + 4 * n_qp_all + op.n_fields() * 8 * n_qp_all
-                     + (8 * n_cell if op.n_fields() == 16 else 0)},
+                     + 8 * n_cell * (19 if gradient_storage else 1)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
     }
     del op, src, dst

@@ -190,6 +190,20 @@ size_t mfg_operator_memory_consumption(mfg_operator op);
  * Jinv (9), JxW, c.  Copies n_fields*n_qp values to the device pointer out;
  * n_fields may be queried with out == NULL. */
 int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields);
+/* Point storage of the tensor2, formulation-1 cache (no counterpart in the
+ * reference, whose storage is fixed; the operator is the same either way).
+ * MFG_T2_FOLDED: Jinv_cur (9), JxW_ref s, JxW_ref tau (6), plus JxW_ref on
+ * meshes with a cell that is not affine -- the layout mfg_operator_qp_data
+ * documents.  MFG_T2_GRADIENT (meshes whose cells are all affine): the
+ * reference-cell gradient of the linearisation point Grad_xi u[d][f] (9) and
+ * s = mu - 2 lambda ln J, 10 words per point; vmult rebuilds Jinv_cur =
+ * (J_ref + Grad_xi u)^{-1} and tau = mu F F^T - s I per point from the
+ * per-cell J_ref.  Default FOLDED.  Changing the storage discards the cache:
+ * call mfg_operator_cache (and compute_diagonal) again. */
+#define MFG_T2_FOLDED 0
+#define MFG_T2_GRADIENT 1
+int mfg_operator_set_point_storage(mfg_operator op, int32_t storage);
+int32_t mfg_operator_point_storage(mfg_operator op);
 /* Number of cell colours of the device layout.  Cells are stored
  * colour-major; cells of one colour share no node, so each colour is one
  * launch whose scatter needs no atomics (deterministic results).  Replaces the
@@ -431,9 +445,10 @@ int64_t mfg_session_n_dofs(const mfg_session_spec *spec);
 #define MFG_BATCH_SOLID_F1 0 /* tensor2, formulation 1 */
 #define MFG_BATCH_SOLID_F0 1 /* tensor2, formulation 0 */
 #define MFG_BATCH_HEAT 2
+#define MFG_BATCH_SOLID_F1_GRADIENT 3 /* tensor2, formulation 1, MFG_T2_GRADIENT point storage */
 typedef struct mfg_batch_tuning {
   int32_t schedule;
-  int32_t cells_per_task[3][7]; /* [kind][degree] */
+  int32_t cells_per_task[4][7]; /* [kind][degree] */
   int32_t waves_per_simd;       /* register budget of the kernel (2 or 4) */
   double ms_colour, ms_queue;   /* one application of the whole batch */
 } mfg_batch_tuning;

@@ -163,6 +163,8 @@ class NeoHookOperator : public OperatorBase<Number> {
   void cache(const DeviceVector<UNumber> &displacement) {
     check(mfg_solid_cache(this->op_, displacement.data(), precision_of<UNumber>()));
   }
+  // MFG_T2_FOLDED or MFG_T2_GRADIENT (tensor2, formulation 1); call cache() again afterwards
+  void set_point_storage(const int32_t storage) { check(mfg_operator_set_point_storage(this->op_, storage)); }
 };
 
 // LaplaceOperator<dim, number>: M + dt c K.

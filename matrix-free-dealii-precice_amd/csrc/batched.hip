@@ -24,14 +24,17 @@
 
 namespace mfg {
 
+static_assert(kSolidF1 == MFG_BATCH_SOLID_F1 && kSolidF0 == MFG_BATCH_SOLID_F0 && kHeat == MFG_BATCH_HEAT &&
+                  kSolidF1G == MFG_BATCH_SOLID_F1_GRADIENT && kNumKinds == 4,
+              "instance kinds are the ABI's MFG_BATCH_* numbers");
 static int code_of(const OperatorData &op) {
-  const int kind = op.kind == 1 ? kHeat : (op.sp.formulation == 1 ? kSolidF1 : kSolidF0);
+  const int kind = op.kind == 1 ? kHeat : op.t2_gradient() ? kSolidF1G : (op.sp.formulation == 1 ? kSolidF1 : kSolidF0);
   return kind * 8 + op.p;
 }
 static int max_cpb(int p) { return kBatchThreads / ((p + 1) * (p + 1) * (p + 1)); }
 
 struct Tuning {
-  int cpb[24] = {};  // per code; 0 = the largest that fits kBatchThreads
+  int cpb[8 * kNumKinds] = {};  // per code; 0 = the largest that fits kBatchThreads
   int queue = 1;
   int waves = 2;     // register budget of the kernel: 2 or 4 waves per SIMD (FP32 only)
   double ms_colour = 0, ms_queue = 0;
@@ -117,12 +120,14 @@ struct ApplyGroup {
       e.prm[0] = e.prm[1] = e.prm[2] = Number(0);
       e.cell_jxw = op.affine_ref && op.kind == 0 ? op.cell_jxw.as<Number>() : nullptr;
       e.cell_geo = op.affine_ref && op.kind == 1 ? op.cell_geo.as<Number>() : nullptr;
+      e.cell_jac = op.t2_gradient() ? op.cell_jac.as<Number>() : nullptr;
       e.wq = op.wq3.as<Number>();
       if (op.kind == 1) {
         e.prm[0] = Number(op.dt);
       } else if (op.sp.formulation == 1) {
         e.prm[0] = Number(2.0 * op.lambda);
         e.prm[1] = Number(op.rho_alpha);
+        e.prm[2] = Number(op.sp.mu);
       } else {
         e.prm[0] = Number(op.kappa);
         e.prm[1] = Number(op.c_1);
@@ -132,7 +137,7 @@ struct ApplyGroup {
     }
     upload_vec(inst, hi, s);
     kinds.clear();
-    for (int kind : {kSolidF1, kSolidF0, kHeat}) {
+    for (int kind : {kSolidF1, kSolidF1G, kSolidF0, kHeat}) {
       bool present = false;
       for (int i = 0; i < n; ++i) present |= (hi[i].code >> 3) == kind;
       if (!present) continue;
@@ -232,6 +237,8 @@ struct ApplyGroup {
     for (const KindTasks &kt : kinds) {
       if (kt.kind == kSolidF1)
         launch_kind<Number, WAVES, kind_bit(kSolidF1)>(kt, s, vecs, active, fix_mode, assign, queue, error);
+      else if (kt.kind == kSolidF1G)
+        launch_kind<Number, WAVES, kind_bit(kSolidF1G)>(kt, s, vecs, active, fix_mode, assign, queue, error);
       else if (kt.kind == kSolidF0)
         launch_kind<Number, WAVES, kind_bit(kSolidF0)>(kt, s, vecs, active, fix_mode, assign, queue, error);
       else
@@ -1135,7 +1142,7 @@ static void export_tuning(const Tuning &tu, mfg_batch_tuning *out) {
   if (!out) return;
   std::memset(out, 0, sizeof(*out));
   out->schedule = tu.queue ? MFG_SCHEDULE_QUEUE : MFG_SCHEDULE_COLOUR;
-  for (int kind = 0; kind < 3; ++kind)
+  for (int kind = 0; kind < kNumKinds; ++kind)
     for (int p = 1; p <= kMaxDegree; ++p) out->cells_per_task[kind][p] = tu.cpb_for(kind * 8 + p);
   out->waves_per_simd = tu.waves;
   out->ms_colour = tu.ms_colour;
@@ -1181,7 +1188,7 @@ int mfg_batch_set_schedule(mfg_batch b, int32_t schedule) {
 int mfg_batch_set_cells_per_task(mfg_batch b, int32_t kind, int32_t degree, int32_t cells) {
   return guarded([&] {
     MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
-    MFG_REQUIRE(kind >= 0 && kind < 3 && degree >= 1 && degree <= kMaxDegree, MFG_ERR_ARGUMENT,
+    MFG_REQUIRE(kind >= 0 && kind < kNumKinds && degree >= 1 && degree <= kMaxDegree, MFG_ERR_ARGUMENT,
                 "unknown kind or degree");
     MFG_REQUIRE(cells >= 0 && cells <= max_cpb(degree), MFG_ERR_ARGUMENT, "cells per task out of range");
     b->b->tu.cpb[kind * 8 + degree] = cells;

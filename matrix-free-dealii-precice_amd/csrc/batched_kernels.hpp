@@ -31,8 +31,10 @@ namespace mfg {
 constexpr int kBatchThreads = 512;
 constexpr int kSpinLimit = 1 << 22;
 
-// instance kinds in the dispatch code (code = kind * 8 + p)
-enum : int { kSolidF1 = 0, kSolidF0 = 1, kHeat = 2 };
+// instance kinds in the dispatch code (code = kind * 8 + p); kSolidF1G is
+// formulation 1 from the 10-word point storage
+enum : int { kSolidF1 = 0, kSolidF0 = 1, kHeat = 2, kSolidF1G = 3 };
+constexpr int kNumKinds = 4;
 constexpr int kind_bit(int kind) { return 1 << kind; }
 
 template <typename Number>
@@ -44,9 +46,10 @@ struct BatchInst {
   int64_t nqp, n_constrained;
   int code, cpb;
   int id;  // batch-wide instance number (active mask)
-  Number prm[3];  // F1: 2 lambda, rho alpha_1; F0: kappa, c_1, rho alpha_1; heat: dt
+  Number prm[3];  // F1: 2 lambda, rho alpha_1 (and mu for F1G); F0: kappa, c_1, rho alpha_1; heat: dt
   const Number *cell_jxw, *wq;  // F1: per-cell reference JxW (or null) and tensor weights
   const Number *cell_geo;       // heat: per-cell Jinv and det J (or null)
+  const Number *cell_jac;       // F1G: per-cell J_ref and J_ref^{-1}
 };
 
 struct BatchTask {
@@ -103,6 +106,12 @@ __device__ inline void batch_task_degree(int kind, const BatchInst<Number> &in, 
   if constexpr ((KINDS & kind_bit(kSolidF1)) != 0) {
     if (kind == kSolidF1) {
       QSolidT2F1<Number> q{in.qd, in.nqp, in.prm[0], in.prm[1], in.cell_jxw, in.wq};
+      batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
+    }
+  }
+  if constexpr ((KINDS & kind_bit(kSolidF1G)) != 0) {
+    if (kind == kSolidF1G) {
+      QSolidT2F1G<Number> q{in.qd, in.nqp, in.prm[2], in.prm[0], in.prm[1], in.cell_jac, in.cell_jxw, in.wq};
       batch_task_cells<P>(q, in, v, tk, assign, s_tab, s_tile);
     }
   }

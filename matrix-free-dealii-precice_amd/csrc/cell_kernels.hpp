@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace mfg {
 
@@ -206,6 +207,9 @@ __device__ inline Number stream_load(const Number *p) {
 // reference gradients at a point into submitted value and reference gradient
 // already multiplied by JxW.
 
+// symmetric index pairs xx, yy, zz, xy, xz, yz
+constexpr int kSymI[6] = {0, 1, 2, 0, 0, 1}, kSymJ[6] = {0, 1, 2, 1, 2, 2};
+
 // Solid, tensor2 caching, formulation 1 (mf_nh_operator.h:945-974).
 // The reference caches, per point, the current-frame Jinv and JxW (from
 // MappingQEulerian) and the scalars 2 s / J, 1 / J and tau / J, where
@@ -266,6 +270,100 @@ struct QSolidT2F1 {
 #pragma unroll
       for (int f = 0; f < 3; ++f)
         grad[c][f] = Pm[c][0] * d.ji[f * 3 + 0] + Pm[c][1] * d.ji[f * 3 + 1] + Pm[c][2] * d.ji[f * 3 + 2];
+      val[c] = rho_alpha * d.w * val[c];
+    }
+  }
+};
+
+// Solid, tensor2 caching, formulation 1, rebuilt per point from the
+// displacement gradient: the operator of QSolidT2F1 from 10 words per point.
+// On affine meshes J_ref and J_ref^{-1} are constant per cell, so what the
+// folded layout stores follows from Grad_xi u (the reference-cell gradient of
+// the linearisation point, 9 words) and s = mu - 2 lambda ln J (1 word) with
+// the arithmetic of the cache kernel:
+//   J_cur = J_ref + Grad_xi u,   Jinv_cur = J_cur^{-1},
+//   F = I + Grad_xi u J_ref^{-1},   tau = mu F F^T - s I,
+// and JxW_ref = det J_ref w_q.  Fields: Grad_xi u [d][f] (9), s.
+template <typename Number>
+struct QSolidT2F1G {
+  static constexpr int NC = 3;
+  static constexpr int kFields = 10;
+  static constexpr bool kNodalField = false;
+  const Number *qd;
+  int64_t nqp;
+  Number mu, two_lambda, rho_alpha;
+  const Number *cell_jac;  // per stored cell: J_ref (9), J_ref^{-1} (9)
+  const Number *cell_jxw;  // det J_ref per stored cell
+  const Number *wq;        // N^3 tensor-product quadrature weights
+  struct Data {
+    Number gu[9], s, w;
+    const Number *jac;
+  };
+  __device__ Data load(int64_t q, int cell, int pt) const {
+    Data d;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) d.gu[f] = stream_load(qd + f * nqp + q);
+    d.s = stream_load(qd + 9 * nqp + q);
+    d.w = cell_jxw[cell] * wq[pt];
+    d.jac = cell_jac + (int64_t)cell * 18;
+    return d;
+  }
+  __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
+    const Number *Jr = d.jac, *Jri = d.jac + 9;
+    Number ji[9];
+    {
+      Number Jc[3][3];
+#pragma unroll
+      for (int f = 0; f < 9; ++f) Jc[f / 3][f % 3] = Jr[f] + d.gu[f];
+      const Number id =
+          Number(1) / (Jc[0][0] * (Jc[1][1] * Jc[2][2] - Jc[1][2] * Jc[2][1]) -
+                       Jc[0][1] * (Jc[1][0] * Jc[2][2] - Jc[1][2] * Jc[2][0]) + Jc[0][2] * (Jc[1][0] * Jc[2][1] - Jc[1][1] * Jc[2][0]));
+      ji[0] = (Jc[1][1] * Jc[2][2] - Jc[1][2] * Jc[2][1]) * id;
+      ji[1] = (Jc[0][2] * Jc[2][1] - Jc[0][1] * Jc[2][2]) * id;
+      ji[2] = (Jc[0][1] * Jc[1][2] - Jc[0][2] * Jc[1][1]) * id;
+      ji[3] = (Jc[1][2] * Jc[2][0] - Jc[1][0] * Jc[2][2]) * id;
+      ji[4] = (Jc[0][0] * Jc[2][2] - Jc[0][2] * Jc[2][0]) * id;
+      ji[5] = (Jc[0][2] * Jc[1][0] - Jc[0][0] * Jc[1][2]) * id;
+      ji[6] = (Jc[1][0] * Jc[2][1] - Jc[1][1] * Jc[2][0]) * id;
+      ji[7] = (Jc[0][1] * Jc[2][0] - Jc[0][0] * Jc[2][1]) * id;
+      ji[8] = (Jc[0][0] * Jc[1][1] - Jc[0][1] * Jc[1][0]) * id;
+    }
+    Number T[6];
+    {
+      Number F[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+          F[a][e] = d.gu[a * 3 + 0] * Jri[0 * 3 + e] + d.gu[a * 3 + 1] * Jri[1 * 3 + e] + d.gu[a * 3 + 2] * Jri[2 * 3 + e] +
+                    (a == e ? Number(1) : Number(0));
+#pragma unroll
+      for (int s6 = 0; s6 < 6; ++s6) {
+        const int x = kSymI[s6], y = kSymJ[s6];
+        const Number b = F[x][0] * F[y][0] + F[x][1] * F[y][1] + F[x][2] * F[y][2];
+        T[s6] = d.w * (mu * b - (x == y ? d.s : Number(0)));
+      }
+    }
+    const Number sw = d.w * d.s;
+    Number g[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) g[c][e] = grad[c][0] * ji[0 * 3 + e] + grad[c][1] * ji[1 * 3 + e] + grad[c][2] * ji[2 * 3 + e];
+    const Number tr = g[0][0] + g[1][1] + g[2][2];
+    const Number diag = two_lambda * d.w * tr;
+    const Number Tm[3][3] = {{T[0], T[3], T[4]}, {T[3], T[1], T[5]}, {T[4], T[5], T[2]}};
+    Number Pm[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        Pm[a][b] = sw * (g[a][b] + g[b][a]) + (a == b ? diag : Number(0)) + g[a][0] * Tm[0][b] + g[a][1] * Tm[1][b] +
+                   g[a][2] * Tm[2][b];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) grad[c][f] = Pm[c][0] * ji[f * 3 + 0] + Pm[c][1] * ji[f * 3 + 1] + Pm[c][2] * ji[f * 3 + 2];
       val[c] = rho_alpha * d.w * val[c];
     }
   }
@@ -405,7 +503,6 @@ struct QHeat {
 // The other formulation-1 cachings (MF caching parameter,
 // mf_nh_operator.h:848-1010).  They define the same operator as tensor2 and
 // trade memory for arithmetic differently.
-constexpr int kSymI[6] = {0, 1, 2, 0, 0, 1}, kSymJ[6] = {0, 1, 2, 1, 2, 2};
 
 // scalar_referential (mf_nh_operator.h:848-943): reference-frame geometry
 // and one scalar per point; F is rebuilt from the cached nodal displacement
@@ -665,10 +762,12 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   // scatter at the end is a plain store.  The first colour touching a node
   // stores without reading.  Read with the gather, the load's latency hides
   // behind the sweeps; read after them, its 2 NC registers are free during
-  // the sweeps.  For FP64 and p <= 4 the late read gives the kernel one more
-  // wave per SIMD and is the faster of the two (measured); elsewhere the
-  // early read is.
-  constexpr bool kLateDst = sizeof(Number) == 8 && P <= 4;
+  // the sweeps.  For the FP64 solid ops with per-point tensors and p <= 4 the
+  // late read gives the kernel one more wave per SIMD and is the faster of the
+  // two (measured); for the heat op and the 10-word tensor2 op, whose wave
+  // count it does not change, and elsewhere the early read is.
+  constexpr bool kLateDst = sizeof(Number) == 8 && P <= 4 && !std::is_same<QOp, QHeat<Number>>::value &&
+                            !std::is_same<QOp, QSolidT2F1G<Number>>::value;
   const bool first = assign && (entry >> 31);
   Number dold[NC];
   if constexpr (!kLateDst && !kToCells) {

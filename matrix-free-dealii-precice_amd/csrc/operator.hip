@@ -31,6 +31,15 @@ const Number *t2_cell_jxw(const OperatorData &op) {
   return op.affine_ref ? op.cell_jxw.as<Number>() : nullptr;
 }
 
+// tensor2 formulation 1 from the 10-word storage (affine meshes only)
+template <typename Number>
+QSolidT2F1G<Number> t2_gradient_op(const OperatorData &op) {
+  return QSolidT2F1G<Number>{op.qd.as<Number>(),         op.nqp,
+                             Number(op.sp.mu),           Number(2.0 * op.lambda),
+                             Number(op.rho_alpha),       op.cell_jac.as<Number>(),
+                             op.cell_jxw.as<Number>(),   op.wq3.as<Number>()};
+}
+
 // per-cell geometry of the heat op on affine meshes, or null (per-point fields)
 template <typename Number>
 const Number *heat_cell_geo(const OperatorData &op) {
@@ -151,6 +160,8 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
     } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
       QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
       launch_any(q);
+    } else if (op.t2_gradient()) {
+      launch_any(t2_gradient_op<Number>(op));
     } else if (op.sp.formulation == 1) {
       QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), ra, t2_cell_jxw<Number>(op),
                            op.wq3.as<Number>()};
@@ -185,6 +196,7 @@ static void launch_cache(OperatorData &op, const void *u, int u_fp32) {
   a.formulation = op.sp.formulation;
   a.caching = op.sp.caching;
   a.jxw_ref_field = op.affine_ref ? 0 : 1;
+  a.t2_gradient = op.t2_gradient() ? 1 : 0;
   a.error = op.flag.as<int>();
   const int grid = blocks_for(op.n_cells, CPB);
   cache_solid<P, CPB, Number><<<grid, block_threads<P, CPB>(), 0, op.stream>>>(a);
@@ -216,6 +228,8 @@ static void launch_diagonal(OperatorData &op) {
       } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
         QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
         diagonal_cells<P, Number, QSolidT4NS<Number>><<<grid, NT, 0, op.stream>>>(a, q);
+      } else if (op.t2_gradient()) {
+        diagonal_cells<P, Number, QSolidT2F1G<Number>><<<grid, NT, 0, op.stream>>>(a, t2_gradient_op<Number>(op));
       } else if (op.sp.formulation == 1) {
         QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha),
                              t2_cell_jxw<Number>(op), op.wq3.as<Number>()};
@@ -629,7 +643,7 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
   }
   // affine cells: the mixed terms of the trilinear map vanish (vertex v at
   // (v & 1, v >> 1 & 1, v >> 2 & 1)); then det J_ref is one number per cell
-  std::vector<double> det_ref((size_t)n_cells);
+  std::vector<double> det_ref((size_t)n_cells), jac((size_t)n_cells * 18);
   affine_ref = true;
   for (int c = 0; c < n_cells; ++c) {
     const double *X = &vert[(size_t)c * 24];
@@ -646,9 +660,22 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
                         std::fabs(V(7, e) - V(6, e) - V(5, e) + V(4, e) - V(3, e) + V(2, e) + V(1, e) - V(0, e))});
     }
     if (mixed > 1e-12 * h) affine_ref = false;
-    det_ref[c] = std::fabs(J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    det_ref[c] = std::fabs(det);
+    // J_ref and J_ref^{-1} of an affine cell (the tensor2 gradient storage)
+    double *G = &jac[(size_t)c * 18];
+    for (int f = 0; f < 9; ++f) G[f] = J[f / 3][f % 3];
+    const double id = det != 0 ? 1.0 / det : 0.0;
+    G[9] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+    G[10] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    G[11] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    G[12] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    G[13] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    G[14] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    G[15] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    G[16] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    G[17] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
   }
   upload(cell_nodes, map.data(), map.size(), stream);
   upload(constrained, d.constrained_dofs, (size_t)n_constrained, stream);
@@ -675,10 +702,15 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
     if (precision == MFG_FP64) {
       upload(cell_jxw, det_ref.data(), det_ref.size(), stream);
       upload(wq3, w3.data(), w3.size(), stream);
+      if (kind == 0) upload(cell_jac, jac.data(), jac.size(), stream);
     } else {
       std::vector<float> a(det_ref.begin(), det_ref.end()), b(w3.begin(), w3.end());
       upload(cell_jxw, a.data(), a.size(), stream);
       upload(wq3, b.data(), b.size(), stream);
+      if (kind == 0) {
+        std::vector<float> g(jac.begin(), jac.end());
+        upload(cell_jac, g.data(), g.size(), stream);
+      }
     }
   }
 }
@@ -774,6 +806,7 @@ void OperatorData::setup_solid() {
     case MFG_CACHING_TENSOR4_NS: n_fields = QSolidT4NS<double>::kFields; break;
     default:
       n_fields = sp.formulation == 0 ? QSolidT2F0<double>::kFields
+                 : t2_gradient()     ? QSolidT2F1G<double>::kFields
                  : affine_ref        ? QSolidT2F1<double>::kFields - 1
                                      : QSolidT2F1<double>::kFields;
   }
@@ -782,7 +815,7 @@ void OperatorData::setup_solid() {
 }
 
 size_t OperatorData::memory_consumption() const {
-  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes + cell_jxw.bytes + wq3.bytes + cell_geo.bytes +
+  return qd.bytes + cell_nodes.bytes + constrained.bytes + u_nodal.bytes + cell_jxw.bytes + wq3.bytes + cell_geo.bytes + cell_jac.bytes +
          gather_offsets.bytes + gather_points.bytes + gather_cmask.bytes + contrib.bytes;
 }
 
@@ -986,5 +1019,25 @@ int mfg_operator_set_cell_schedule(mfg_operator op, int32_t schedule) {
 }
 
 int32_t mfg_operator_cell_schedule(mfg_operator op) { return op ? op->cell_schedule : -1; }
+
+int mfg_operator_set_point_storage(mfg_operator op, int32_t storage) {
+  return guarded([&] {
+    MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
+    MFG_REQUIRE(storage == MFG_T2_FOLDED || storage == MFG_T2_GRADIENT, MFG_ERR_ARGUMENT, "unknown point storage");
+    if (storage == op->t2_storage) return;
+    MFG_REQUIRE(op->kind == 0 && op->sp.caching == MFG_CACHING_TENSOR2 && op->sp.formulation == 1, MFG_ERR_ARGUMENT,
+                "the point storage applies to the tensor2 caching of formulation 1");
+    MFG_REQUIRE(storage == MFG_T2_FOLDED || op->affine_ref, MFG_ERR_ARGUMENT,
+                "the gradient storage needs a mesh whose cells are all affine");
+    MFG_HIP(hipStreamSynchronize(op->stream));
+    op->t2_storage = storage;
+    op->setup_solid();  // reallocates the point cache for the new field count
+    op->ready = false;
+    op->has_diagonal = false;
+    ++op->buffers_version;
+  });
+}
+
+int32_t mfg_operator_point_storage(mfg_operator op) { return op ? op->t2_storage : -1; }
 
 }  // extern "C"

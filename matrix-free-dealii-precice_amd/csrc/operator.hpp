@@ -35,6 +35,9 @@ struct OperatorData {
   bool affine_ref = false;
   DeviceBuffer cell_jxw, wq3;
   DeviceBuffer cell_geo;  // heat on affine cells: Jinv (9), det J per stored cell
+  DeviceBuffer cell_jac;  // solid on affine cells: J_ref (9), J_ref^{-1} (9) per stored cell
+  int t2_storage = MFG_T2_FOLDED;  // tensor2 formulation-1 point storage (MFG_T2_*)
+  bool t2_gradient() const { return kind == 0 && sp.caching == MFG_CACHING_TENSOR2 && sp.formulation == 1 && t2_storage == MFG_T2_GRADIENT; }
   DeviceBuffer flag;         // int error flag
   std::vector<int32_t> host_constrained;
   std::vector<int32_t> host_cell_nodes;  // input cell order (used to build transfers)

@@ -22,6 +22,7 @@ struct CacheArgs {
   int formulation;
   int caching;  // MFG_CACHING_*
   int jxw_ref_field;  // tensor2 formulation 1: store JxW_ref as field 16 (not supplied per cell)
+  int t2_gradient;    // tensor2 formulation 1: store Grad_xi u and s (10 fields) instead
   int *error;   // set to 1 if det F <= 0 anywhere
 };
 
@@ -138,6 +139,12 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
   const int64_t n = a.nqp;
   const double wq = a.weights[T.i] * a.weights[T.j] * a.weights[T.k];
   const Number scalar = a.mu - Number(2) * a.lambda * log(J);
+  if (a.t2_gradient) {  // the 10-word tensor2 storage: Grad_xi u [d][f], s
+#pragma unroll
+    for (int f = 0; f < 9; ++f) a.qd[f * n + qidx] = gr[f / 3][f % 3];
+    a.qd[9 * n + qidx] = scalar;
+    return;
+  }
   if (a.caching == MFG_CACHING_SCALAR_REFERENTIAL || a.caching == MFG_CACHING_TENSOR4_NS) {
     // reference-frame geometry (MappingQ1 of the reference configuration)
     const double detr = Jr[0][0] * (Jr[1][1] * Jr[2][2] - Jr[1][2] * Jr[2][1]) -

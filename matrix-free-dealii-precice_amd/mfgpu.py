@@ -18,6 +18,7 @@ FP64, FP32 = 0, 1
 CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
 CELLS_COLOURS, CELLS_BUNDLES, CELLS_PENCILS, CELLS_GATHER = 0, 1, 2, 3
 NUMBERING = {"lexicographic": 0, "cellwise": 1}
+T2_FOLDED, T2_GRADIENT = 0, 1  # tensor2 formulation-1 point storage
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
 _c_dp = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +83,8 @@ SIGNATURES = {
     "mfg_operator_memory_consumption": (ctypes.c_size_t, [_vp]),
     "mfg_operator_qp_data": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32)]),
     "mfg_operator_n_colours": (ctypes.c_int32, [_vp]),
+    "mfg_operator_set_point_storage": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mfg_operator_point_storage": (ctypes.c_int32, [_vp]),
     "mfg_operator_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mfg_operator_timing": (ctypes.c_int, [_vp, _c_dp, ctypes.POINTER(ctypes.c_int64)]),
     "mfg_box_mesh_cells": (ctypes.c_int, [_vp, _c_i32p]),
@@ -297,6 +300,13 @@ class _Operator:
 
     def cell_schedule(self):
         return lib().mfg_operator_cell_schedule(self.h)
+
+    def set_point_storage(self, storage):
+        """T2_FOLDED (16 words per point) or T2_GRADIENT (10 words, affine meshes); discards the cache."""
+        check(lib().mfg_operator_set_point_storage(self.h, storage))
+
+    def point_storage(self):
+        return lib().mfg_operator_point_storage(self.h)
 
     def n_fields(self):
         """Words stored per quadrature point by the operator's cache."""
@@ -618,15 +628,15 @@ class SolidProblem:
 
 # ---------------------------------------------------- batched instances ---
 SCHEDULE_COLOUR, SCHEDULE_QUEUE = 0, 1
-BATCH_SOLID_F1, BATCH_SOLID_F0, BATCH_HEAT = 0, 1, 2
+BATCH_SOLID_F1, BATCH_SOLID_F0, BATCH_HEAT, BATCH_SOLID_F1_GRADIENT = 0, 1, 2, 3
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("schedule", ctypes.c_int32), ("cells_per_task", (ctypes.c_int32 * 7) * 3),
+    _fields_ = [("schedule", ctypes.c_int32), ("cells_per_task", (ctypes.c_int32 * 7) * 4),
                 ("waves_per_simd", ctypes.c_int32), ("ms_colour", ctypes.c_double), ("ms_queue", ctypes.c_double)]
 
     def as_dict(self):
-        kinds = ("solid_f1", "solid_f0", "heat")
+        kinds = ("solid_f1", "solid_f0", "heat", "solid_f1_gradient")
         return {"schedule": "queue" if self.schedule == SCHEDULE_QUEUE else "colour",
                 "cells_per_task": {k: list(self.cells_per_task[i])[1:] for i, k in enumerate(kinds)},
                 "waves_per_simd": self.waves_per_simd, "ms_colour": self.ms_colour, "ms_queue": self.ms_queue}

@@ -14,14 +14,17 @@ import mfgpu
 pytestmark = pytest.mark.gpu
 
 
-def make_ops(specs, precision):
-    """specs: (kind, n_cells, degree, formulation); solids cached at u0."""
+def make_ops(specs, precision, storage=mfgpu.T2_FOLDED):
+    """specs: (kind, n_cells, degree, formulation); solids cached at u0, the
+    formulation-1 ones with the given tensor2 point storage."""
     ops, meshes = [], []
     for k, (kind, n, p, form) in enumerate(specs):
         if kind == "solid":
             m = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], n, p, 3, 1, 7)
             op = mfgpu.NeoHookOperator(m, 384615.38461538462, 0.3, 1000.0, 4e6, formulation=form,
                                        precision=precision)
+            if form == 1:
+                op.set_point_storage(storage)
             u = m.u0(amp=1e-3, noise=1e-5, seed=k)
             op.cache(torch.from_numpy(u).to("cuda", op.dtype))
         else:
@@ -40,11 +43,12 @@ SPECS = [("heat", [3, 4, 2], 1, 1), ("solid", [2, 3, 2], 2, 1), ("heat", [4, 2, 
          ("heat", [2, 2, 1], 4, 1), ("solid", [2, 2, 2], 3, 0), ("heat", [5, 3, 2], 2, 1)]
 
 
-@pytest.mark.parametrize("precision", [mfgpu.FP64, mfgpu.FP32])
+@pytest.mark.parametrize("precision,storage", [(mfgpu.FP64, mfgpu.T2_FOLDED), (mfgpu.FP32, mfgpu.T2_FOLDED),
+                                               (mfgpu.FP64, mfgpu.T2_GRADIENT)])
 @pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE])
-def test_batched_vmult_equals_per_instance(cuda, precision, schedule):
-    print(f"batched vmult precision={precision} schedule={schedule}", flush=True)
-    ops, meshes = make_ops(SPECS, precision)
+def test_batched_vmult_equals_per_instance(cuda, precision, storage, schedule):
+    print(f"batched vmult precision={precision} storage={storage} schedule={schedule}", flush=True)
+    ops, meshes = make_ops(SPECS, precision, storage)
     dt = ops[0].dtype
     srcs = [mfgpu.fill_uniform(100 + i, torch.empty(m.n_dofs, dtype=dt, device="cuda")) for i, m in enumerate(meshes)]
     ref = [torch.empty_like(s) for s in srcs]

@@ -60,7 +60,26 @@ def folded_tensor2(ref, with_jxw_ref):
 @pytest.mark.parametrize("p", [2, 4])
 def test_solid_cache_matches_oracle(cuda, p):
     gm, om, go, oo, u = solid_pair([0, 0, 0], [1.0, 0.6, 0.4], [2, 2, 1], p, amp=1e-1)
-    assert go.n_fields() == 16  # box cells are affine: JxW_ref comes per cell
+    # the gradient storage: Grad_xi u and s = mu - 2 lambda ln J per point.
+    # (J_ref + Grad_xi u)^-1 is the oracle's current-frame Jinv, and s is its
+    # c2 / (2 c1) with c2 = 2 s / J, c1 = 1 / J.
+    assert go.point_storage() == mfgpu.T2_FOLDED
+    go.set_point_storage(mfgpu.T2_GRADIENT)
+    go.cache(dev(u))
+    assert go.n_fields() == 10
+    g = go.qp_data().cpu().numpy()
+    ref18 = oo.tensor2_data()
+    Jref = np.diag([0.5, 0.3, 0.4])
+    Jc = Jref[None, :, :] + g[:9].T.reshape(-1, 3, 3)
+    Jinv = np.linalg.inv(Jc).reshape(-1, 9).T
+    for f in range(9):
+        assert np.abs(Jinv[f] - ref18[f]).max() <= 1e-12 * np.abs(ref18[f]).max(), f
+    s_ref = ref18[10] / (2 * ref18[11])
+    assert np.abs(g[9] - s_ref).max() <= 1e-12 * np.abs(s_ref).max()
+    # the folded storage: 16 words, JxW_ref per cell
+    go.set_point_storage(mfgpu.T2_FOLDED)
+    go.cache(dev(u))
+    assert go.n_fields() == 16
     qd = go.qp_data().cpu().numpy()
     ref = folded_tensor2(oo.tensor2_data(), False)
     for f in range(16):
@@ -385,3 +404,91 @@ def test_error_paths(cuda):
     assert e.value.code == 3 and "det_F" in str(e.value)
     with pytest.raises(mfgpu.MfgError):  # formulation 0 exists for tensor2 only (mf_nh_operator.h:706)
         mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, formulation=0, caching="tensor4")
+
+
+@pytest.mark.parametrize("p,precision,n", [(1, mfgpu.FP64, [4, 3, 2]), (2, mfgpu.FP64, [3, 2, 3]),
+                                           (4, mfgpu.FP64, [4, 2, 2]), (5, mfgpu.FP64, [2, 2, 2]),
+                                           (3, mfgpu.FP32, [3, 3, 3]), (4, mfgpu.FP32, [4, 2, 2])])
+def test_gradient_point_storage(cuda, p, precision, n):
+    """The 10-word tensor2 storage (Grad_xi u and s per point, the rest rebuilt
+    from per-cell J_ref) gives the folded layout's operator: vmult and diagonal
+    against the oracle at the usual bars and against the folded storage to
+    1e-14 (FP64), on every cell schedule; switching back reproduces the folded
+    results bit for bit."""
+    print(f"gradient point storage p={p} precision={precision} n={n}", flush=True)
+    lo, hi = [0, 0, 0], [1.0, 0.6, 0.4]
+    gm = mfgpu.BoxMesh(lo, hi, n, p, 3, 1 | 8, 7, numbering="cellwise")
+    lex = mfgpu.BoxMesh(lo, hi, n, p, 3, 1 | 8, 7)
+    om = O.Mesh(lo, hi, n, p, 3, 1 | 8, 7)
+    perm = np.full(lex.n_nodes, -1, dtype=np.int64)
+    perm[lex.cell_nodes()] = gm.cell_nodes()
+    dof = (perm[:, None] * 3 + np.arange(3)[None, :]).ravel()  # cellwise dof of each lexicographic dof
+    u = om.u0(amp=2e-2, noise=1e-4)
+    oo = O.Solid(om, caching=1)
+    oo.cache(u)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, precision=precision)
+    ug = np.empty(om.n_dofs)
+    ug[dof] = u
+    x = O.fill_uniform(5, om.n_dofs)
+    xg = np.empty(om.n_dofs)
+    xg[dof] = x
+    tol = TOL64 if precision == mfgpu.FP64 else TOL32
+    ref, dref = oo.vmult(x), oo.diagonal()
+    res = {}
+    for storage in (mfgpu.T2_FOLDED, mfgpu.T2_GRADIENT, mfgpu.T2_FOLDED):
+        go.set_point_storage(storage)
+        assert go.point_storage() == storage
+        go.cache(dev(ug, go.dtype))
+        assert go.n_fields() == (10 if storage == mfgpu.T2_GRADIENT else 16)
+        go.compute_diagonal()
+        d = go.diagonal()
+        assert rel(d.double().cpu().numpy()[dof], dref) < tol
+        ys = {}
+        for sched in (mfgpu.CELLS_COLOURS, mfgpu.CELLS_GATHER, mfgpu.CELLS_BUNDLES):
+            go.set_cell_schedule(sched)
+            y = torch.full((om.n_dofs,), float("nan"), dtype=go.dtype, device="cuda")
+            go.vmult(y, dev(xg, go.dtype))
+            assert rel(y.double().cpu().numpy()[dof], ref) < tol, sched
+            ys[sched] = y
+        if storage in res:
+            assert torch.equal(d, res[storage][0])
+            assert all(torch.equal(ys[k], res[storage][1][k]) for k in ys)
+        res[storage] = (d, ys)
+    if precision == mfgpu.FP64:
+        fd, fy = res[mfgpu.T2_FOLDED]
+        gd, gy = res[mfgpu.T2_GRADIENT]
+        assert rel(gd, fd.cpu().numpy()) < 1e-14
+        for k in fy:
+            assert rel(gy[k], fy[k].cpu().numpy()) < 1e-14, k
+
+
+def test_gradient_point_storage_errors(cuda):
+    lo, hi, n, p = [0, 0, 0], [1.0, 0.6, 0.4], [3, 2, 2], 2
+    gm = mfgpu.BoxMesh(lo, hi, n, p, 3, 1, 7)
+    go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
+    assert go.point_storage() == mfgpu.T2_FOLDED  # the default
+    u = torch.from_numpy(gm.u0(amp=1e-2, noise=1e-5)).cuda()
+    go.cache(u)
+    mem16 = go.memory_consumption()
+    go.set_point_storage(mfgpu.T2_GRADIENT)
+    y = torch.zeros(gm.n_dofs, dtype=torch.float64, device="cuda")
+    with pytest.raises(mfgpu.MfgError) as e:  # the change discards the cache
+        go.vmult(y, y)
+    assert e.value.code == 5
+    go.cache(u)
+    assert go.memory_consumption() < mem16
+    with pytest.raises(mfgpu.MfgError):
+        go.set_point_storage(7)
+    # non-affine cells keep the folded storage
+    om = O.Mesh(lo, hi, n, p, 3, 1, 7)
+    dm = DistortedMesh(gm, distort(om.cell_vertices()))
+    dop = mfgpu.NeoHookOperator(dm, O.MU, O.NU, O.RHO, O.ALPHA1)
+    assert dop.point_storage() == mfgpu.T2_FOLDED
+    with pytest.raises(mfgpu.MfgError):
+        dop.set_point_storage(mfgpu.T2_GRADIENT)
+    # other cachings, formulation 0 and heat have no such storage
+    for kw in ({"caching": "tensor4"}, {"formulation": 0}):
+        with pytest.raises(mfgpu.MfgError):
+            mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1, **kw).set_point_storage(mfgpu.T2_GRADIENT)
+    with pytest.raises(mfgpu.MfgError):
+        mfgpu.LaplaceOperator(mfgpu.BoxMesh(lo, hi, n, p, 1, 63, 1), 0.1).set_point_storage(mfgpu.T2_GRADIENT)
