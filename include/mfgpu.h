@@ -94,6 +94,10 @@ int mfg_box_mesh_describe(mfg_mesh mesh, mfg_level_desc *desc);
 int mfg_box_mesh_node_coords(mfg_mesh mesh, double *out);
 /* Cells per direction. */
 int mfg_box_mesh_cells(mfg_mesh mesh, int32_t n_cells[3]);
+/* (cell, face) pairs of a box mesh on boundary id b: 2 d for the low and
+ * 2 d + 1 for the high side in direction d (deal.II's colorized box ids), the
+ * local face number being b itself; faces == NULL returns the count only. */
+int mfg_box_mesh_boundary_faces(mfg_mesh mesh, int32_t boundary_id, int32_t *faces, int64_t *n_faces);
 /* Node numbering of a box mesh (the vector layout in HBM).  A new mesh is
  * LEXICOGRAPHIC.  CELLWISE numbers nodes cell by cell, as
  * DoFHandler::distribute_dofs does (mf_elasticity.h:667, heat_transfer.h:437;
@@ -204,6 +208,24 @@ int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields);
 #define MFG_T2_GRADIENT 1
 int mfg_operator_set_point_storage(mfg_operator op, int32_t storage);
 int32_t mfg_operator_point_storage(mfg_operator op);
+/* Coupling interface of the solid (Solid::assemble_residual,
+ * mf_elasticity.h:1849-1897): rhs += int t0 . w dA over the listed faces,
+ * with the spatial traction t pulled back at the displacement u as
+ * t0 = |det F F^{-T} N| t.  faces: host array of n_faces pairs (input cell
+ * index, local face 2 axis + side, side 1 = the cell's high side in reference
+ * direction axis).  traction: device pointer, 3 values per face point, point
+ * q = qa + (p+1) qb over the Gauss points of the face's two tangential
+ * reference directions in ascending order (what preCICE's read gives per
+ * quadrature point, mf_elasticity.h:1876-1878).  u, rhs: FP64 device vectors
+ * of the operator's layout; constrained dofs are left as they are (the
+ * reference zeroes them afterwards, :1907).  FP64 solid operators only.
+ * Faces share edge nodes: the sums use atomics (order-dependent rounding). */
+int mfg_solid_add_interface_traction(mfg_operator op, int64_t n_faces, const int32_t *faces, const double *u,
+                                     const double *traction, double *rhs);
+/* Reference-configuration coordinates of the same face points (the coupling
+ * mesh the reference writes to preCICE, include/adapter/quad_interface.h:
+ * 198-340): 3 doubles per point into the device pointer xyz. */
+int mfg_operator_face_quadrature_points(mfg_operator op, int64_t n_faces, const int32_t *faces, double *xyz);
 /* Number of cell colours of the device layout.  Cells are stored
  * colour-major; cells of one colour share no node, so each colour is one
  * launch whose scatter needs no atomics (deterministic results).  Replaces the

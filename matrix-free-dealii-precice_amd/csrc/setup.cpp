@@ -261,6 +261,30 @@ int mfg_box_mesh_fill_u0(mfg_mesh m, double amp, double noise, uint64_t seed, do
   });
 }
 
+int mfg_box_mesh_boundary_faces(mfg_mesh m, int32_t boundary_id, int32_t *faces, int64_t *n_faces) {
+  return guarded([&] {
+    MFG_REQUIRE(m && n_faces, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(boundary_id >= 0 && boundary_id < 6, MFG_ERR_ARGUMENT, "box boundary ids are 0..5");
+    const int axis = boundary_id / 2, side = boundary_id % 2;
+    const int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
+    const int64_t count = (int64_t)m->n[a] * m->n[b];
+    if (faces) {
+      int64_t f = 0;
+      for (int jb = 0; jb < m->n[b]; ++jb)
+        for (int ja = 0; ja < m->n[a]; ++ja) {
+          int cc[3];
+          cc[axis] = side ? m->n[axis] - 1 : 0;
+          cc[a] = ja;
+          cc[b] = jb;
+          faces[2 * f] = (int32_t)(cc[0] + (int64_t)m->n[0] * (cc[1] + (int64_t)m->n[1] * cc[2]));
+          faces[2 * f + 1] = boundary_id;  // local face 2 axis + side: the same number on a box
+          ++f;
+        }
+    }
+    *n_faces = count;
+  });
+}
+
 int mfg_box_mesh_cells(mfg_mesh m, int32_t n[3]) {
   return guarded([&] {
     MFG_REQUIRE(m && n, MFG_ERR_ARGUMENT, "null argument");

@@ -88,6 +88,9 @@ SIGNATURES = {
     "mfg_operator_enable_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "mfg_operator_timing": (ctypes.c_int, [_vp, _c_dp, ctypes.POINTER(ctypes.c_int64)]),
     "mfg_box_mesh_cells": (ctypes.c_int, [_vp, _c_i32p]),
+    "mfg_box_mesh_boundary_faces": (ctypes.c_int, [_vp, ctypes.c_int32, _c_i32p, ctypes.POINTER(ctypes.c_int64)]),
+    "mfg_solid_add_interface_traction": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp, _vp, _vp]),
+    "mfg_operator_face_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp]),
     "mfg_box_mesh_children": (ctypes.c_int, [_vp, _vp, _c_i32p]),
     "mfg_transfer_create": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.POINTER(_vp)]),
     "mfg_transfer_destroy": (ctypes.c_int, [_vp]),
@@ -187,6 +190,15 @@ class BoxMesh:
         import numpy as np
         n = self.n_cells * (self.degree + 1) ** 3
         return np.ctypeslib.as_array(self.desc.cell_nodes, shape=(n,)).copy()
+
+    def boundary_faces(self, boundary_id):
+        """(cell, face) pairs on box boundary id 2 d (low side) / 2 d + 1 (high side) of direction d, int32 (n, 2)."""
+        import numpy as np
+        n = ctypes.c_int64(0)
+        check(lib().mfg_box_mesh_boundary_faces(self.h, boundary_id, None, ctypes.byref(n)))
+        out = np.zeros(2 * n.value, dtype=np.int32)
+        check(lib().mfg_box_mesh_boundary_faces(self.h, boundary_id, out.ctypes.data_as(_c_i32p), ctypes.byref(n)))
+        return out.reshape(-1, 2)
 
     def constrained(self):
         import numpy as np
@@ -337,6 +349,23 @@ class NeoHookOperator(_Operator):
         import torch
         prec = FP64 if u.dtype == torch.float64 else FP32
         check(lib().mfg_solid_cache(self.h, _ptr(u), prec))
+
+    def add_interface_traction(self, rhs, u, faces, traction):
+        """rhs += int t0 . w dA over faces ((n, 2) int32 cell/face pairs), t0 = |det F F^-T N| t at u;
+        traction: FP64 device tensor, 3 values per face point ((p+1)^2 points per face)."""
+        import numpy as np
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        check(lib().mfg_solid_add_interface_traction(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(u),
+                                                     _ptr(traction), _ptr(rhs)))
+
+    def face_quadrature_points(self, faces):
+        """Reference coordinates of the face points, (n_faces * (p+1)^2, 3) FP64 device tensor."""
+        import numpy as np
+        import torch
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        out = torch.empty((len(f) // 2) * (self._keep.degree + 1) ** 2 * 3, dtype=torch.float64, device="cuda")
+        check(lib().mfg_operator_face_quadrature_points(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(out)))
+        return out.view(-1, 3)
 
 
 class LaplaceOperator(_Operator):

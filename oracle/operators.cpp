@@ -788,6 +788,74 @@ void orc_solid_tensor2_data(void *op, double *out) {
   }
 }
 
+void orc_interface_traction(void *mp, int64_t n_faces, const int32_t *faces, const double *u, const double *t,
+                            double *rhs) {
+  // mf_elasticity.h:1849-1897: t0 = |det F F^{-T} N| t on the reference face.
+  // Restated through Nanson's formula, |det F F^{-T} N| dA = |x_a x x_b| dxi_a
+  // dxi_b with x = X + u and a, b the face's tangential reference directions:
+  // the deformed area element straight from the deformed tangents, with no
+  // F^{-1} and no normal.
+  const Mesh &m = *(const Mesh *)mp;
+  const int N = m.N, N3 = N * N * N;
+  vector<double> gq(N), gw(N), val(N), der(N);
+  gauss_rule(N, gq.data(), gw.data());
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const int cell = faces[2 * f], face = faces[2 * f + 1];
+    const int axis = face / 2, side = face % 2;
+    const int ta = axis == 0 ? 1 : 0, tb = axis == 2 ? 1 : 2;
+    const int32_t *cn = &m.cell_nodes[(size_t)cell * N3];
+    for (int qb = 0; qb < N; ++qb)
+      for (int qa = 0; qa < N; ++qa) {
+        double xi[3];
+        xi[axis] = side;
+        xi[ta] = gq[qa];
+        xi[tb] = gq[qb];
+        double v[3][16], d[3][16];
+        for (int e = 0; e < 3; ++e) {
+          lagrange_eval(N, m.gll.data(), xi[e], val.data(), der.data());
+          for (int a = 0; a < N; ++a) {
+            v[e][a] = val[a];
+            d[e][a] = der[a];
+          }
+        }
+        double J[3][3];
+        q1_jacobian(&m.vert[(size_t)cell * 24], xi, J);
+        double xa[3], xb[3];
+        for (int c = 0; c < 3; ++c) {
+          xa[c] = J[c][ta];
+          xb[c] = J[c][tb];
+        }
+        for (int k = 0; k < N; ++k)
+          for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+              const int ijk[3] = {i, j, k};
+              double da = 1, db = 1;
+              for (int e = 0; e < 3; ++e) {
+                da *= e == ta ? d[e][ijk[e]] : v[e][ijk[e]];
+                db *= e == tb ? d[e][ijk[e]] : v[e][ijk[e]];
+              }
+              const int64_t node = cn[i + N * (j + N * k)];
+              for (int c = 0; c < 3; ++c) {
+                xa[c] += u[node * 3 + c] * da;
+                xb[c] += u[node * 3 + c] * db;
+              }
+            }
+        const double cr[3] = {xa[1] * xb[2] - xa[2] * xb[1], xa[2] * xb[0] - xa[0] * xb[2], xa[0] * xb[1] - xa[1] * xb[0]};
+        const double area = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * gw[qa] * gw[qb];
+        const double *tq = t + ((size_t)f * N * N + qa + (size_t)N * qb) * 3;
+        for (int k = 0; k < N; ++k)
+          for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+              const double phi = v[0][i] * v[1][j] * v[2][k];
+              if (phi == 0.0) continue;
+              const int64_t node = cn[i + N * (j + N * k)];
+              for (int c = 0; c < 3; ++c)
+                if (!m.is_con[node * 3 + c]) rhs[node * 3 + c] += tq[c] * phi * area;
+            }
+      }
+  }
+}
+
 void orc_solid_residual(void *op, const double *u, const double *acc, const double *body, double *R) {
   // mf_elasticity.h:1764-1846 (fast FEEvaluation path) + set_zero (1905)
   Solid &s = *(Solid *)op;

@@ -95,6 +95,14 @@ void orc_solid_tensor2_data(void *op, double *out);
 void orc_solid_residual(void *op, const double *u, const double *acc,
                         const double *body, double *R);
 
+/* Interface traction (mf_elasticity.h:1849-1897): rhs += int t0 . w dA over
+ * n_faces (cell, face = 2 axis + side) pairs, t0 = |det F F^{-T} N| t at the
+ * displacement u, computed through Nanson's formula from the deformed
+ * tangents; t: 3 per face point, point qa + N qb over the Gauss points of the
+ * two tangential directions (ascending); constrained dofs untouched. */
+void orc_interface_traction(void *mesh, int64_t n_faces, const int32_t *faces, const double *u, const double *t,
+                            double *rhs);
+
 /* ------------------------------------------------------- heat operator --- */
 /* A = M + dt * c(x) K on scalar FE_Q(p) (heat_transfer.h:234-250).
  * coef: per (cell, q) conductivity table, or NULL for c == 1.

@@ -74,6 +74,7 @@ def lib():
             "orc_cg_solve": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                             ctypes.c_double, ctypes.c_int, _dp]),
             "orc_cheb_eigenvalues": (None, [ctypes.c_int, _vp, _vp, ctypes.c_int, _dp, _dp]),
+            "orc_interface_traction": (None, [_vp, ctypes.c_int64, _ip, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -168,6 +169,29 @@ class Mesh:
         v = np.ascontiguousarray(v, dtype=np.float64)
         assert v.size == self.n_cells * 24
         lib().orc_mesh_set_cell_vertices(self.h, dptr(v))
+
+    def boundary_faces(self, boundary_id):
+        """(cell, face) pairs on box boundary id 2 d (low side) or 2 d + 1 (high side) of direction d."""
+        axis, side = boundary_id // 2, boundary_id % 2
+        a, b = (1, 2) if axis == 0 else ((0, 2) if axis == 1 else (0, 1))
+        n = self.n
+        out = []
+        for jb in range(n[b]):
+            for ja in range(n[a]):
+                cc = [0, 0, 0]
+                cc[axis] = n[axis] - 1 if side else 0
+                cc[a], cc[b] = ja, jb
+                out += [cc[0] + n[0] * (cc[1] + n[1] * cc[2]), boundary_id]
+        return np.array(out, dtype=np.int32)
+
+    def interface_traction(self, faces, u, t):
+        """rhs of the traction term: int t0 . w dA, t0 = |det F F^-T N| t (3 per face point)."""
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        rhs = np.zeros(self.n_dofs)
+        lib().orc_interface_traction(self.h, len(faces) // 2, iptr(faces), dptr(u), dptr(t), dptr(rhs))
+        return rhs
 
     def u0(self, amp=1e-3, noise=1e-5, seed=0):
         u = np.zeros(self.n_dofs)

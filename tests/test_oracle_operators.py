@@ -279,3 +279,44 @@ def test_heat_manufactured_quadratic_solution_is_exact():
         assert it > 0
         u = w + ud
         assert np.abs(u - exact(t)).max() < 1e-11
+
+
+def test_interface_traction_closed_forms():
+    """The oracle's interface traction (Nanson's formula from the deformed
+    tangents) against what the kinematics fix: with u = 0 the traction
+    integrates to t times the face area; a uniform stretch lambda along x keeps
+    the x faces' area and scales the others by lambda; a shear u = g y e_x
+    scales the x faces by sqrt(1 + g^2) and keeps the y faces; the first moment
+    of a constant traction is t times the face's first moment."""
+    lo, hi, n, p = [0, 0, 0], [2.0, 0.6, 0.4], [3, 2, 2], 2
+    m = O.Mesh(lo, hi, n, p, 3, 0, 7)
+    X = m.node_coords()
+    tv = np.array([1.5, -0.7, 0.3])
+    area = {0: 0.6 * 0.4, 1: 0.6 * 0.4, 2: 2.0 * 0.4, 3: 2.0 * 0.4, 4: 2.0 * 0.6, 5: 2.0 * 0.6}
+
+    def total(faces, u):
+        t = np.tile(tv, len(faces) // 2 * (p + 1) ** 2)
+        return m.interface_traction(faces, u, t).reshape(-1, 3)
+
+    zero = np.zeros(m.n_dofs)
+    for bid in range(6):
+        faces = m.boundary_faces(bid)
+        r = total(faces, zero)
+        assert np.allclose(r.sum(axis=0), tv * area[bid], rtol=1e-13, atol=0), bid
+        on_face = np.isclose(X[:, bid // 2], (lo if bid % 2 == 0 else hi)[bid // 2])
+        assert np.all(r[~on_face] == 0.0)
+        # first moment: the nodal basis reproduces linear functions
+        tang = [d for d in range(3) if d != bid // 2][0]
+        centroid = (lo[tang] + hi[tang]) / 2
+        assert np.allclose((r * X[:, tang:tang + 1]).sum(axis=0), tv * area[bid] * centroid, rtol=1e-13, atol=0)
+    lam, g = 1.3, 0.45
+    stretch = np.zeros((m.n_nodes, 3))
+    stretch[:, 0] = (lam - 1) * X[:, 0]
+    shear = np.zeros((m.n_nodes, 3))
+    shear[:, 0] = g * X[:, 1]
+    for bid in range(6):
+        faces = m.boundary_faces(bid)
+        s = 1.0 if bid // 2 == 0 else lam
+        assert np.allclose(total(faces, stretch.ravel()).sum(axis=0), s * tv * area[bid], rtol=1e-13, atol=0), bid
+        s = np.sqrt(1 + g * g) if bid // 2 == 0 else 1.0
+        assert np.allclose(total(faces, shear.ravel()).sum(axis=0), s * tv * area[bid], rtol=1e-13, atol=0), bid
