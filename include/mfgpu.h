@@ -395,6 +395,16 @@ int mfg_solid_problem_get_state(mfg_solid_problem p, double *u, double *v, doubl
  * iterate). */
 int mfg_solid_problem_step(mfg_solid_problem p, mfg_step_info *info);
 /* Device bytes held by the problem (operators, caches, MG, state vectors). */
+/* Coupling interface of the time step: every residual of the following
+ * steps adds the traction term over the faces, pulled back at the current
+ * displacement (as mfg_solid_add_interface_traction; mf_elasticity.h:
+ * 1849-1897).  faces: host array of n_faces (input cell, local face) pairs of
+ * the finest level; traction: device pointer to 3 values per face point,
+ * owned by the caller, read at every residual -- the caller refreshes it
+ * between steps (preCICE read, mf_elasticity.h:1876-1878) and keeps it alive
+ * while steps run.  n_faces = 0 removes the interface. */
+int mfg_solid_problem_set_interface(mfg_solid_problem p, int64_t n_faces, const int32_t *faces,
+                                    const double *traction);
 size_t mfg_solid_problem_memory(mfg_solid_problem p);
 
 /* ---------------------------------------------------------- serving loop --- */

@@ -182,7 +182,7 @@ __global__ void face_points_kernel(const double *__restrict__ vertices, const in
 }
 
 // Validates the (input cell, face) pairs and uploads them as (stored cell, face).
-static DeviceBuffer stored_faces(const OperatorData &op, int64_t n_faces, const int32_t *faces) {
+DeviceBuffer stored_faces(const OperatorData &op, int64_t n_faces, const int32_t *faces) {
   MFG_REQUIRE(n_faces >= 0 && n_faces < (int64_t)1 << 30, MFG_ERR_ARGUMENT, "bad face count");
   MFG_REQUIRE(n_faces == 0 || faces, MFG_ERR_ARGUMENT, "null face list");
   std::vector<int32_t> inv(op.n_cells);
@@ -203,6 +203,14 @@ static DeviceBuffer stored_faces(const OperatorData &op, int64_t n_faces, const 
   return b;
 }
 
+void add_interface_traction(const OperatorData &op, int64_t n_faces, const int32_t *stored, const double *u,
+                            const double *traction, double *rhs) {
+  if (n_faces == 0) return;
+  TractionArgs a{op.cell_nodes.as<uint32_t>(), op.vertices.as<double>(), stored, u, traction, rhs};
+  interface_traction_kernel<<<(int)n_faces, 64, 0, op.stream>>>(a, face_tables(op.p));
+  MFG_HIP(hipGetLastError());
+}
+
 }  // namespace mfg
 
 using namespace mfg;
@@ -218,9 +226,7 @@ int mfg_solid_add_interface_traction(mfg_operator op, int64_t n_faces, const int
     MFG_REQUIRE(u && rhs && (n_faces == 0 || traction), MFG_ERR_ARGUMENT, "null vector");
     DeviceBuffer fb = stored_faces(*op, n_faces, faces);
     if (n_faces == 0) return;
-    TractionArgs a{op->cell_nodes.as<uint32_t>(), op->vertices.as<double>(), fb.as<int32_t>(), u, traction, rhs};
-    interface_traction_kernel<<<(int)n_faces, 64, 0, op->stream>>>(a, face_tables(op->p));
-    MFG_HIP(hipGetLastError());
+    add_interface_traction(*op, n_faces, fb.as<int32_t>(), u, traction, rhs);
     MFG_HIP(hipStreamSynchronize(op->stream));  // the face list is freed on return
   });
 }

@@ -51,9 +51,11 @@ static void residual_launch(const OperatorData &op, const double *u, const doubl
   }
 }
 
-// R = residual(u, acc) with constrained entries zero; returns ||R||
+// R = residual(u, acc) (+ the interface traction over n_faces stored face
+// pairs) with constrained entries zero; returns ||R||
 double solid_residual(const OperatorData &op, const double *u, const double *acc, const double *body, double *R,
-                      Reducer &red) {
+                      Reducer &red, int64_t n_faces = 0, const int32_t *faces = nullptr,
+                      const double *traction = nullptr) {
   MFG_REQUIRE(op.kind == 0 && op.precision == MFG_FP64, MFG_ERR_ARGUMENT, "residual needs an FP64 solid operator");
   vec_fill(MFG_FP64, R, op.n_dofs, 0.0, op.stream);
   vec_fill(MFG_FP32, op.flag.ptr, 1, 0.0, op.stream);  // int 0 == float 0 bit pattern
@@ -66,6 +68,8 @@ double solid_residual(const OperatorData &op, const double *u, const double *acc
     case 6: residual_launch<6>(op, u, acc, body, R); break;
     default: throw Error(MFG_ERR_ARGUMENT, "unsupported degree");
   }
+  // mf_elasticity.h:1849-1897, skipping constrained dofs (zero after :1907)
+  add_interface_traction(op, n_faces, faces, u, traction, R);
   const double n = std::sqrt(red.dot(MFG_FP64, R, R, op.n_dofs, op.stream));
   int h = 0;
   MFG_HIP(hipMemcpyAsync(&h, op.flag.ptr, sizeof(int), hipMemcpyDeviceToHost, op.stream));
@@ -113,7 +117,8 @@ void SolidProblem::step(mfg_step_info &info) {
     newmark_acc_kernel<double><<<grid, 256, 0, stream>>>(a.as<double>(), delta_u.as<double>(), v_old.as<double>(),
                                                          a_old.as<double>(), a1, a2, a3, N);
     MFG_HIP(hipGetLastError());
-    res = solid_residual(F, u.as<double>(), a.as<double>(), nm.body_force, R.as<double>(), red);
+    res = solid_residual(F, u.as<double>(), a.as<double>(), nm.body_force, R.as<double>(), red, iface_n,
+                         iface_faces.as<int32_t>(), iface_traction);
     if (it == 0) {
       res0 = res;
       info.residual_0 = res;
@@ -155,7 +160,8 @@ void SolidProblem::step(mfg_step_info &info) {
 }
 
 size_t SolidProblem::memory() const {
-  size_t b = u.bytes + du.bytes + delta_u.bytes + v.bytes + a.bytes + v_old.bytes + a_old.bytes + R.bytes;
+  size_t b = u.bytes + du.bytes + delta_u.bytes + v.bytes + a.bytes + v_old.bytes + a_old.bytes + R.bytes +
+             iface_faces.bytes;
   for (auto &o : levels) b += o->memory_consumption() + o->diag.bytes + o->inv_diag.bytes + o->vertices.bytes;
   for (auto &L : mg->levels)
     for (const DeviceBuffer *x : {&L.sol, &L.def, &L.tmp, &L.d, &L.t, &L.cg_r, &L.cg_p, &L.cg_h, &L.cg_z}) b += x->bytes;
@@ -320,6 +326,19 @@ int mfg_solid_problem_step(mfg_solid_problem p, mfg_step_info *info) {
   return guarded([&] {
     MFG_REQUIRE(p, MFG_ERR_ARGUMENT, "null problem");
     p->p->step(*info);
+  });
+}
+
+int mfg_solid_problem_set_interface(mfg_solid_problem p, int64_t n_faces, const int32_t *faces,
+                                    const double *traction) {
+  return guarded([&] {
+    MFG_REQUIRE(p, MFG_ERR_ARGUMENT, "null problem");
+    MFG_REQUIRE(n_faces == 0 || traction, MFG_ERR_ARGUMENT, "null traction");
+    SolidProblem &sp = *p->p;
+    DeviceBuffer fb = stored_faces(sp.fine(), n_faces, faces);
+    sp.iface_faces = std::move(fb);
+    sp.iface_n = n_faces;
+    sp.iface_traction = n_faces ? traction : nullptr;
   });
 }
 

@@ -22,6 +22,11 @@ struct SolidProblem {
   hipStream_t stream = nullptr;
   CGWorkspace ws;
   Reducer red;
+  // coupling interface: (stored cell, face) pairs on the device and the
+  // caller's traction per face point, added to every residual
+  DeviceBuffer iface_faces;
+  int64_t iface_n = 0;
+  const double *iface_traction = nullptr;
 
   OperatorData &fine() { return fine64 ? *fine64 : *levels.back(); }
   int64_t n() { return fine().n_dofs; }
@@ -30,6 +35,12 @@ struct SolidProblem {
   void step(mfg_step_info &info);
   size_t memory() const;
 };
+
+// (input cell, face) pairs -> (stored cell, face) pairs on the device
+DeviceBuffer stored_faces(const OperatorData &op, int64_t n_faces, const int32_t *faces);
+// rhs += traction term over the faces (stored pairs on the device), stream-ordered
+void add_interface_traction(const OperatorData &op, int64_t n_faces, const int32_t *stored, const double *u,
+                            const double *traction, double *rhs);
 
 // Builds a problem (levels coarse..fine, children[l-1] between l-1 and l).
 std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_desc *levels,

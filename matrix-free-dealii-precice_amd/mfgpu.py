@@ -520,6 +520,7 @@ SIGNATURES.update({
     "mfg_solid_problem_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "mfg_solid_problem_step": (ctypes.c_int, [_vp, ctypes.POINTER(StepInfo)]),
     "mfg_solid_problem_memory": (ctypes.c_size_t, [_vp]),
+    "mfg_solid_problem_set_interface": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp]),
 })
 
 
@@ -650,6 +651,15 @@ class SolidProblem:
         info = StepInfo()
         check(lib().mfg_solid_problem_step(self.h, ctypes.byref(info)))
         return info
+
+    def set_interface(self, faces, traction):
+        """Traction on the finest level's faces ((n, 2) cell/face pairs), 3 values per face point in an FP64
+        device tensor that the steps read (refresh its contents between steps); faces of length 0 clear it."""
+        import numpy as np
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        self._traction = traction if len(f) else None
+        check(lib().mfg_solid_problem_set_interface(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p),
+                                                    _ptr(traction) if len(f) else None))
 
     def memory(self):
         return lib().mfg_solid_problem_memory(self.h)

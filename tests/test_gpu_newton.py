@@ -56,3 +56,44 @@ def test_newmark_steps_match_oracle(cuda, level_precision):
         R = oops[-1].residual(gu, ga, BODY)
         assert np.linalg.norm(R) <= 1e-8 * info.residual_0 * 10
     assert prob.memory() > 0
+
+
+@pytest.mark.parametrize("level_precision", [mfgpu.FP64, mfgpu.FP32])
+def test_coupled_steps_with_interface_traction(cuda, level_precision):
+    """Newmark steps with a traction on the x = 1 face (the coupling interface
+    of mf_elasticity.h:1849-1897): every converged state satisfies the
+    oracle's equations of motion including its (Nanson-form) traction term;
+    an interface that is set and then removed gives the uncoupled step bit for
+    bit."""
+    lo, hi = [0, 0, 0], [1, 1, 1]
+    gms = [mfgpu.BoxMesh(lo, hi, [1 << l] * 3, 2, 3, 1, 7) for l in range(3)]
+    oms = O.box_levels(lo, hi, [1, 1, 1], 3, 2, 3)
+    oop = O.Solid(oms[-1], caching=1)
+    kw = dict(body=BODY, level_precision=level_precision, tol_f=1e-8,
+              coarse_kind=1 if level_precision == mfgpu.FP32 else 0)
+    faces = gms[-1].boundary_faces(1)
+    t = np.tile([0.0, -5e3, 2e3], len(faces) * 9)
+    tdev = dev(t)
+    u = oms[-1].u0(amp=1e-3, noise=1e-5)
+    zero = np.zeros_like(u)
+    prob = mfgpu.SolidProblem(gms, O.MU, O.NU, O.RHO, **kw)
+    prob.set_interface(faces, tdev)
+    prob.set_state(dev(u), dev(zero), dev(zero))
+    plain = mfgpu.SolidProblem(gms, O.MU, O.NU, O.RHO, **kw)
+    plain.set_state(dev(u), dev(zero), dev(zero))
+    cleared = mfgpu.SolidProblem(gms, O.MU, O.NU, O.RHO, **kw)
+    cleared.set_interface(faces, tdev)
+    cleared.set_interface(np.zeros((0, 2), dtype=np.int32), None)
+    cleared.set_state(dev(u), dev(zero), dev(zero))
+    for step in range(2):
+        info = prob.step()
+        assert info.converged
+        gu, gv, ga = (x.cpu().numpy() for x in prob.get_state())
+        R = oop.residual(gu, ga, BODY) + oms[-1].interface_traction(faces.ravel(), gu, t)
+        print(f"step {step}: newton {info.newton_iterations}, |R| {np.linalg.norm(R):.2e} vs R0 {info.residual_0:.2e}",
+              flush=True)
+        assert np.linalg.norm(R) <= 1e-8 * info.residual_0 * 10
+        pi, ci = plain.step(), cleared.step()
+        pu, cu = plain.get_state()[0], cleared.get_state()[0]
+        assert torch.equal(pu, cu) and pi.newton_iterations == ci.newton_iterations
+        assert float((pu - dev(gu)).norm()) > 1e-3 * float(pu.norm())  # the traction moves the solid
