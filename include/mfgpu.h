@@ -226,6 +226,18 @@ int mfg_solid_add_interface_traction(mfg_operator op, int64_t n_faces, const int
  * mesh the reference writes to preCICE, include/adapter/quad_interface.h:
  * 198-340): 3 doubles per point into the device pointer xyz. */
 int mfg_operator_face_quadrature_points(mfg_operator op, int64_t n_faces, const int32_t *faces, double *xyz);
+/* Values of an FP64 field u of the operator's layout (n_components per node)
+ * at the same face points: n_components doubles per point into the device
+ * pointer values -- the data the reference writes to preCICE at the interface
+ * quadrature points (displacement, quad_interface.h:198-340; temperature). */
+int mfg_operator_face_values(mfg_operator op, int64_t n_faces, const int32_t *faces, const double *u,
+                             double *values);
+/* Heat interface flux (LaplaceProblem::assemble_rhs, heat_transfer.h:618-652):
+ * rhs -= dt int q phi dA over the faces, q = flux per face point (device
+ * pointer, point order as above), dt the operator's; constrained dofs are left
+ * as they are.  FP64 rhs; atomics on shared edge nodes. */
+int mfg_heat_add_interface_flux(mfg_operator op, int64_t n_faces, const int32_t *faces, const double *flux,
+                                double *rhs);
 /* Number of cell colours of the device layout.  Cells are stored
  * colour-major; cells of one colour share no node, so each colour is one
  * launch whose scatter needs no atomics (deterministic results).  Replaces the

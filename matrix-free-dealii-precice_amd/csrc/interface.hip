@@ -158,6 +158,77 @@ __global__ void __launch_bounds__(64) interface_traction_kernel(TractionArgs a, 
   }
 }
 
+// Values of an nc-component field at the face points (what the reference
+// writes to preCICE at the interface quadrature points: the displacement of
+// the solid, the temperature of the heat problem).  Only the face's nodes
+// carry nonzero basis values there (GLL nodes).  One thread per face point.
+__global__ void face_values_kernel(const uint32_t *__restrict__ cell_nodes, const int32_t *__restrict__ faces,
+                                   int n_faces, int nc, FaceTables tb, const double *__restrict__ u,
+                                   double *__restrict__ out) {
+  const int N = tb.N, N2 = N * N, N3 = N2 * N;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)n_faces * N2) return;
+  const int f = (int)(g / N2), q = (int)(g - (int64_t)f * N2), qa = q % N, qb = q / N;
+  int axis, side, ta, tbx;
+  face_axes(faces[2 * f + 1], axis, side, ta, tbx);
+  const uint32_t *map = cell_nodes + (int64_t)faces[2 * f] * N3;
+  double acc[3] = {0, 0, 0};
+  for (int nb = 0; nb < N; ++nb)
+    for (int na = 0; na < N; ++na) {
+      int e3[3];
+      e3[axis] = side ? N - 1 : 0;
+      e3[ta] = na;
+      e3[tbx] = nb;
+      const int64_t node = map[e3[0] + N * (e3[1] + N * e3[2])] & kNodeMaskDev;
+      const double phi = tb.val[qa * N + na] * tb.val[qb * N + nb];
+      for (int c = 0; c < nc; ++c) acc[c] += phi * u[node * nc + c];
+    }
+  for (int c = 0; c < nc; ++c) out[g * nc + c] = acc[c];
+}
+
+// Heat interface flux (LaplaceProblem::assemble_rhs, heat_transfer.h:618-652):
+// rhs -= dt int q phi dA over the faces.  One workgroup per face: the point
+// weights -dt q dA go through LDS to one thread per face node; atomic sums on
+// shared edge nodes; constrained dofs are skipped.
+__global__ void __launch_bounds__(64) interface_flux_kernel(const uint32_t *__restrict__ cell_nodes,
+                                                            const double *__restrict__ vertices,
+                                                            const int32_t *__restrict__ faces, FaceTables tb,
+                                                            const double *__restrict__ flux, double dt,
+                                                            double *__restrict__ rhs) {
+  const int N = tb.N, N2 = N * N, N3 = N2 * N;
+  __shared__ double s_w[kFaceMaxN * kFaceMaxN];
+  const int f = blockIdx.x, t = threadIdx.x;
+  const int cell = faces[2 * f];
+  int axis, side, ta, tbx;
+  face_axes(faces[2 * f + 1], axis, side, ta, tbx);
+  if (t < N2) {
+    const int qa = t % N, qb = t / N;
+    double xi[3];
+    xi[axis] = side;
+    xi[ta] = tb.points[qa];
+    xi[tbx] = tb.points[qb];
+    double J[3][3], Ji[3][3];
+    trilinear_jacobian(vertices + (int64_t)cell * 24, xi[0], xi[1], xi[2], J);
+    inv3(J, Ji);
+    const double mn = sqrt(Ji[axis][0] * Ji[axis][0] + Ji[axis][1] * Ji[axis][1] + Ji[axis][2] * Ji[axis][2]);
+    const double dA = fabs(det3(J)) * mn * tb.weights[qa] * tb.weights[qb];
+    s_w[t] = -dt * flux[(int64_t)f * N2 + t] * dA;
+  }
+  __syncthreads();
+  if (t < N2) {
+    const int na = t % N, nb = t / N;
+    int e3[3];
+    e3[axis] = side ? N - 1 : 0;
+    e3[ta] = na;
+    e3[tbx] = nb;
+    const uint32_t entry = cell_nodes[(int64_t)cell * N3 + e3[0] + N * (e3[1] + N * e3[2])];
+    double acc = 0;
+    for (int qb = 0; qb < N; ++qb)
+      for (int qa = 0; qa < N; ++qa) acc += s_w[qa + N * qb] * tb.val[qa * N + na] * tb.val[qb * N + nb];
+    if (!((entry >> kConstraintShift) & 1u)) atomicAdd(&rhs[entry & kNodeMaskDev], acc);
+  }
+}
+
 // Reference coordinates of the face points, point q = qa + N qb.
 __global__ void face_points_kernel(const double *__restrict__ vertices, const int32_t *__restrict__ faces, int n_faces,
                                    FaceTables tb, double *__restrict__ xyz) {
@@ -240,6 +311,36 @@ int mfg_operator_face_quadrature_points(mfg_operator op, int64_t n_faces, const 
     const int64_t n = n_faces * tb.N * tb.N;
     face_points_kernel<<<blocks_for(n, 256), 256, 0, op->stream>>>(op->vertices.as<double>(), fb.as<int32_t>(),
                                                                     (int)n_faces, tb, xyz);
+    MFG_HIP(hipGetLastError());
+    MFG_HIP(hipStreamSynchronize(op->stream));
+  });
+}
+
+int mfg_operator_face_values(mfg_operator op, int64_t n_faces, const int32_t *faces, const double *u,
+                             double *values) {
+  return guarded([&] {
+    MFG_REQUIRE(op && u && (values || n_faces == 0), MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(op->nc <= 3, MFG_ERR_ARGUMENT, "at most 3 components");
+    DeviceBuffer fb = stored_faces(*op, n_faces, faces);
+    if (n_faces == 0) return;
+    const FaceTables tb = face_tables(op->p);
+    const int64_t n = n_faces * tb.N * tb.N;
+    face_values_kernel<<<blocks_for(n, 256), 256, 0, op->stream>>>(op->cell_nodes.as<uint32_t>(), fb.as<int32_t>(),
+                                                                    (int)n_faces, op->nc, tb, u, values);
+    MFG_HIP(hipGetLastError());
+    MFG_HIP(hipStreamSynchronize(op->stream));
+  });
+}
+
+int mfg_heat_add_interface_flux(mfg_operator op, int64_t n_faces, const int32_t *faces, const double *flux,
+                                double *rhs) {
+  return guarded([&] {
+    MFG_REQUIRE(op && rhs && (flux || n_faces == 0), MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(op->kind == 1, MFG_ERR_ARGUMENT, "the interface flux belongs to the heat operator");
+    DeviceBuffer fb = stored_faces(*op, n_faces, faces);
+    if (n_faces == 0) return;
+    interface_flux_kernel<<<(int)n_faces, 64, 0, op->stream>>>(op->cell_nodes.as<uint32_t>(), op->vertices.as<double>(),
+                                                               fb.as<int32_t>(), face_tables(op->p), flux, op->dt, rhs);
     MFG_HIP(hipGetLastError());
     MFG_HIP(hipStreamSynchronize(op->stream));
   });

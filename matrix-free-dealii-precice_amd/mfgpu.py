@@ -91,6 +91,8 @@ SIGNATURES = {
     "mfg_box_mesh_boundary_faces": (ctypes.c_int, [_vp, ctypes.c_int32, _c_i32p, ctypes.POINTER(ctypes.c_int64)]),
     "mfg_solid_add_interface_traction": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp, _vp, _vp]),
     "mfg_operator_face_quadrature_points": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp]),
+    "mfg_operator_face_values": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp, _vp]),
+    "mfg_heat_add_interface_flux": (ctypes.c_int, [_vp, ctypes.c_int64, _c_i32p, _vp, _vp]),
     "mfg_box_mesh_children": (ctypes.c_int, [_vp, _vp, _c_i32p]),
     "mfg_transfer_create": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.POINTER(_vp)]),
     "mfg_transfer_destroy": (ctypes.c_int, [_vp]),
@@ -306,6 +308,25 @@ class _Operator:
         check(lib().mfg_operator_quadrature_points(self.h, _ptr(out)))
         return out.view(self.n_qp, 3)
 
+    def face_quadrature_points(self, faces):
+        """Reference coordinates of the face points, (n_faces * (p+1)^2, 3) FP64 device tensor."""
+        import numpy as np
+        import torch
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        out = torch.empty((len(f) // 2) * (self._keep.degree + 1) ** 2 * 3, dtype=torch.float64, device="cuda")
+        check(lib().mfg_operator_face_quadrature_points(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(out)))
+        return out.view(-1, 3)
+
+    def face_values(self, faces, u):
+        """Values of the FP64 field u at the face points, (n_faces * (p+1)^2, n_components) device tensor."""
+        import numpy as np
+        import torch
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        nc = self._keep.n_components
+        out = torch.empty((len(f) // 2) * (self._keep.degree + 1) ** 2 * nc, dtype=torch.float64, device="cuda")
+        check(lib().mfg_operator_face_values(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(u), _ptr(out)))
+        return out.view(-1, nc)
+
     def set_cell_schedule(self, schedule):
         """CELLS_COLOURS or CELLS_BUNDLES (structured boxes; falls back to colours elsewhere)."""
         check(lib().mfg_operator_set_cell_schedule(self.h, schedule))
@@ -358,14 +379,7 @@ class NeoHookOperator(_Operator):
         check(lib().mfg_solid_add_interface_traction(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(u),
                                                      _ptr(traction), _ptr(rhs)))
 
-    def face_quadrature_points(self, faces):
-        """Reference coordinates of the face points, (n_faces * (p+1)^2, 3) FP64 device tensor."""
-        import numpy as np
-        import torch
-        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
-        out = torch.empty((len(f) // 2) * (self._keep.degree + 1) ** 2 * 3, dtype=torch.float64, device="cuda")
-        check(lib().mfg_operator_face_quadrature_points(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(out)))
-        return out.view(-1, 3)
+
 
 
 class LaplaceOperator(_Operator):
@@ -380,6 +394,13 @@ class LaplaceOperator(_Operator):
                                     ctypes.byref(h)))
         super().__init__(h, precision, mesh)
         self.n_qp = mesh.n_cells * (mesh.degree + 1) ** 3
+
+    def add_interface_flux(self, rhs, faces, flux):
+        """rhs -= dt int q phi dA over faces ((n, 2) cell/face pairs), q: FP64 device tensor per face point."""
+        import numpy as np
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        check(lib().mfg_heat_add_interface_flux(self.h, len(f) // 2, f.ctypes.data_as(_c_i32p), _ptr(flux),
+                                                _ptr(rhs)))
 
     def assemble_rhs(self, rhs, u_old, u_dirichlet, f_qp=None):
         """LaplaceProblem::assemble_rhs: rhs = int (u_old + dt f) phi - A_unconstrained u_D, constrained rows 0."""

@@ -856,6 +856,86 @@ void orc_interface_traction(void *mp, int64_t n_faces, const int32_t *faces, con
   }
 }
 
+// Face points of (cell, face = 2 axis + side): reference coordinates xi of
+// point qa + N qb over the Gauss points of the two tangential directions.
+static void face_point_xi(int face, int qa, int qb, const vector<double> &gq, double xi[3], int &ta, int &tb) {
+  const int axis = face / 2, side = face % 2;
+  ta = axis == 0 ? 1 : 0;
+  tb = axis == 2 ? 1 : 2;
+  xi[axis] = side;
+  xi[ta] = gq[qa];
+  xi[tb] = gq[qb];
+}
+
+void orc_face_values(void *mp, int64_t n_faces, const int32_t *faces, const double *u, double *out) {
+  // full tensor-product evaluation over all cell nodes (FEFaceEvaluation
+  // values at the interface quadrature points, quad_interface.h:198-340)
+  const Mesh &m = *(const Mesh *)mp;
+  const int N = m.N, N3 = N * N * N, nc = m.nc;
+  vector<double> gq(N), gw(N), val(N), der(N);
+  gauss_rule(N, gq.data(), gw.data());
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const int32_t *cn = &m.cell_nodes[(size_t)faces[2 * f] * N3];
+    for (int qb = 0; qb < N; ++qb)
+      for (int qa = 0; qa < N; ++qa) {
+        double xi[3];
+        int ta, tb;
+        face_point_xi(faces[2 * f + 1], qa, qb, gq, xi, ta, tb);
+        double v[3][16];
+        for (int e = 0; e < 3; ++e) {
+          lagrange_eval(N, m.gll.data(), xi[e], val.data(), der.data());
+          for (int a = 0; a < N; ++a) v[e][a] = val[a];
+        }
+        double *o = out + ((size_t)f * N * N + qa + (size_t)N * qb) * nc;
+        for (int c = 0; c < nc; ++c) o[c] = 0;
+        for (int k = 0; k < N; ++k)
+          for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+              const double phi = v[0][i] * v[1][j] * v[2][k];
+              const int64_t node = cn[i + N * (j + N * k)];
+              for (int c = 0; c < nc; ++c) o[c] += phi * u[node * nc + c];
+            }
+      }
+  }
+}
+
+void orc_interface_flux(void *mp, int64_t n_faces, const int32_t *faces, const double *flux, double dt, double *rhs) {
+  // heat_transfer.h:618-652: submit -q dt on the face, integrate; the area
+  // element from the cross product of the face's reference tangents
+  const Mesh &m = *(const Mesh *)mp;
+  const int N = m.N, N3 = N * N * N;
+  vector<double> gq(N), gw(N), val(N), der(N);
+  gauss_rule(N, gq.data(), gw.data());
+  for (int64_t f = 0; f < n_faces; ++f) {
+    const int cell = faces[2 * f];
+    const int32_t *cn = &m.cell_nodes[(size_t)cell * N3];
+    for (int qb = 0; qb < N; ++qb)
+      for (int qa = 0; qa < N; ++qa) {
+        double xi[3];
+        int ta, tb;
+        face_point_xi(faces[2 * f + 1], qa, qb, gq, xi, ta, tb);
+        double J[3][3];
+        q1_jacobian(&m.vert[(size_t)cell * 24], xi, J);
+        const double a[3] = {J[0][ta], J[1][ta], J[2][ta]}, b[3] = {J[0][tb], J[1][tb], J[2][tb]};
+        const double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const double w = -dt * flux[(size_t)f * N * N + qa + (size_t)N * qb] *
+                         std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * gw[qa] * gw[qb];
+        double v[3][16];
+        for (int e = 0; e < 3; ++e) {
+          lagrange_eval(N, m.gll.data(), xi[e], val.data(), der.data());
+          for (int k = 0; k < N; ++k) v[e][k] = val[k];
+        }
+        for (int k = 0; k < N; ++k)
+          for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+              const double phi = v[0][i] * v[1][j] * v[2][k];
+              const int64_t node = cn[i + N * (j + N * k)];
+              if (phi != 0.0 && !m.is_con[node * m.nc]) rhs[node * m.nc] += w * phi;
+            }
+      }
+  }
+}
+
 void orc_solid_residual(void *op, const double *u, const double *acc, const double *body, double *R) {
   // mf_elasticity.h:1764-1846 (fast FEEvaluation path) + set_zero (1905)
   Solid &s = *(Solid *)op;

@@ -103,6 +103,14 @@ void orc_solid_residual(void *op, const double *u, const double *acc,
 void orc_interface_traction(void *mesh, int64_t n_faces, const int32_t *faces, const double *u, const double *t,
                             double *rhs);
 
+/* Field values (mesh components per node) at the face points, point order as
+ * orc_interface_traction; out: n_faces * N^2 * nc. */
+void orc_face_values(void *mesh, int64_t n_faces, const int32_t *faces, const double *u, double *out);
+/* Heat interface flux (heat_transfer.h:618-652): rhs -= dt int q phi dA, q per
+ * face point; constrained dofs untouched (scalar meshes). */
+void orc_interface_flux(void *mesh, int64_t n_faces, const int32_t *faces, const double *flux, double dt,
+                        double *rhs);
+
 /* ------------------------------------------------------- heat operator --- */
 /* A = M + dt * c(x) K on scalar FE_Q(p) (heat_transfer.h:234-250).
  * coef: per (cell, q) conductivity table, or NULL for c == 1.

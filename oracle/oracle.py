@@ -75,6 +75,8 @@ def lib():
                                             ctypes.c_double, ctypes.c_int, _dp]),
             "orc_cheb_eigenvalues": (None, [ctypes.c_int, _vp, _vp, ctypes.c_int, _dp, _dp]),
             "orc_interface_traction": (None, [_vp, ctypes.c_int64, _ip, _dp, _dp, _dp]),
+            "orc_face_values": (None, [_vp, ctypes.c_int64, _ip, _dp, _dp]),
+            "orc_interface_flux": (None, [_vp, ctypes.c_int64, _ip, _dp, ctypes.c_double, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -191,6 +193,22 @@ class Mesh:
         t = np.ascontiguousarray(t, dtype=np.float64)
         rhs = np.zeros(self.n_dofs)
         lib().orc_interface_traction(self.h, len(faces) // 2, iptr(faces), dptr(u), dptr(t), dptr(rhs))
+        return rhs
+
+    def face_values(self, faces, u):
+        """Field values at the face points, (n_faces * N^2, nc)."""
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(len(faces) // 2 * (self.p + 1) ** 2 * self.nc)
+        lib().orc_face_values(self.h, len(faces) // 2, iptr(faces), dptr(u), dptr(out))
+        return out.reshape(-1, self.nc)
+
+    def interface_flux(self, faces, flux, dt):
+        """Heat rhs term -dt int q phi dA (q per face point)."""
+        faces = np.ascontiguousarray(faces, dtype=np.int32)
+        flux = np.ascontiguousarray(flux, dtype=np.float64)
+        rhs = np.zeros(self.n_dofs)
+        lib().orc_interface_flux(self.h, len(faces) // 2, iptr(faces), dptr(flux), dt, dptr(rhs))
         return rhs
 
     def u0(self, amp=1e-3, noise=1e-5, seed=0):

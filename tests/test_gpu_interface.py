@@ -105,3 +105,40 @@ def test_traction_errors(cuda):
         op.add_interface_traction(u, u, np.array([[0, 6]]), t)  # no face 6
     with pytest.raises(mfgpu.MfgError):
         gm.boundary_faces(6)
+
+
+@pytest.mark.parametrize("p,n", [(1, [3, 2, 2]), (2, [2, 3, 2]), (4, [2, 2, 1]), (6, [1, 1, 2])])
+def test_face_values_and_heat_flux_match_oracle(cuda, p, n):
+    """Face values of the solid displacement and of a temperature field, and
+    the heat interface flux term, against the oracle (full tensor-product
+    evaluation; cross-product area element) at relative 1e-13 / 1e-12."""
+    print(f"face values / heat flux p={p} n={n}", flush=True)
+    om = O.Mesh(LO, HI, n, p, 3, 1, 7)
+    gm = mfgpu.BoxMesh(LO, HI, n, p, 3, 1, 7)
+    sop = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
+    u = om.u0(amp=5e-2, noise=1e-3)
+    hm = O.Mesh(LO, HI, n, p, 1, 63 - 2, 1)  # every face but x = 1 Dirichlet
+    hg = mfgpu.BoxMesh(LO, HI, n, p, 1, 63 - 2, 1)
+    dt = 0.05
+    hop = mfgpu.LaplaceOperator(hg, dt)
+    T = O.fill_uniform(7, hm.n_dofs)
+    con = hm.constrained()
+    for bid in range(6):
+        faces = gm.boundary_faces(bid)
+        ref = om.face_values(faces.ravel(), u)
+        got = sop.face_values(faces, dev(u)).cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), bid
+        ref = hm.face_values(faces.ravel(), T)
+        got = hop.face_values(faces, dev(T)).cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), bid
+        q = O.fill_uniform(30 + bid, len(faces) * (p + 1) ** 2)
+        ref = hm.interface_flux(faces.ravel(), q, dt)
+        base = O.fill_uniform(40 + bid, hm.n_dofs)
+        rhs = dev(base)
+        hop.add_interface_flux(rhs, faces, dev(q))
+        got = rhs.cpu().numpy() - base
+        if np.linalg.norm(ref) > 0:
+            assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref), bid
+        else:
+            assert np.abs(got).max() <= 1e-15
+        assert np.array_equal(rhs.cpu().numpy()[con], base[con])

@@ -320,3 +320,46 @@ def test_interface_traction_closed_forms():
         assert np.allclose(total(faces, stretch.ravel()).sum(axis=0), s * tv * area[bid], rtol=1e-13, atol=0), bid
         s = np.sqrt(1 + g * g) if bid // 2 == 0 else 1.0
         assert np.allclose(total(faces, shear.ravel()).sum(axis=0), s * tv * area[bid], rtol=1e-13, atol=0), bid
+
+
+def test_face_values_and_heat_flux_closed_forms():
+    """Face values reproduce a field the element space holds (a trilinear-free
+    linear function, exact at every face point); a constant heat flux q
+    integrates to -dt q times the face area, with the face's first moment."""
+    lo, hi, n, p = [0, 0, 0], [2.0, 0.6, 0.4], [3, 2, 2], 3
+    N = p + 1
+    gauss = (np.polynomial.legendre.leggauss(N)[0] + 1) / 2
+    h = (np.array(hi) - np.array(lo)) / n
+    for nc in (1, 3):
+        m = O.Mesh(lo, hi, n, p, nc, 0, 7)
+        X = m.node_coords()
+        coef = np.array([[0.5, 2.0, -1.0, 3.0], [1.0, -0.5, 0.25, 0.0], [-2.0, 0.0, 1.5, 1.0]])[:nc]
+        u = np.stack([c[0] + X @ c[1:] for c in coef], axis=1).ravel()
+        for bid in range(6):
+            faces = m.boundary_faces(bid)
+            vals = m.face_values(faces, u)
+            axis, side = bid // 2, bid % 2
+            ta, tb = [d for d in range(3) if d != axis]
+            pts = []
+            for cell in faces[0::2]:
+                cc = [cell % n[0], (cell // n[0]) % n[1], cell // (n[0] * n[1])]
+                for q in range(N * N):
+                    x = np.empty(3)
+                    x[axis] = lo[axis] + (cc[axis] + side) * h[axis]
+                    x[ta] = lo[ta] + (cc[ta] + gauss[q % N]) * h[ta]
+                    x[tb] = lo[tb] + (cc[tb] + gauss[q // N]) * h[tb]
+                    pts.append(x)
+            pts = np.array(pts)
+            want = np.stack([c[0] + pts @ c[1:] for c in coef], axis=1)
+            assert np.abs(vals - want).max() <= 1e-13 * np.abs(want).max(), (nc, bid)
+    m = O.Mesh(lo, hi, n, p, 1, 0, 1)
+    X = m.node_coords()
+    area = {0: 0.24, 1: 0.24, 2: 0.8, 3: 0.8, 4: 1.2, 5: 1.2}
+    q, dt = 2.5, 0.1
+    for bid in range(6):
+        faces = m.boundary_faces(bid)
+        r = m.interface_flux(faces, np.full(len(faces) // 2 * N * N, q), dt)
+        assert abs(r.sum() + dt * q * area[bid]) <= 1e-13 * dt * q * area[bid]
+        tang = [d for d in range(3) if d != bid // 2][0]
+        cen = (lo[tang] + hi[tang]) / 2
+        assert abs(r @ X[:, tang] + dt * q * area[bid] * cen) <= 1e-13 * dt * q * area[bid]
