@@ -201,6 +201,17 @@ __device__ inline Number stream_load(const Number *p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Per-cell tables (geometry of affine cells) are read-only during a launch.
+// Read through the constant address space: where the cell index is uniform
+// across the wave (one cell per workgroup) the loads become scalar loads into
+// SGPRs, otherwise they stay vector loads.
+template <typename Number>
+using ConstantPtr = const Number __attribute__((address_space(4))) *;
+template <typename Number>
+__device__ inline ConstantPtr<Number> as_constant(const Number *p) {
+  return (ConstantPtr<Number>)p;
+}
+
 // ------------------------------------------------------ quadrature ops ---
 // Each op provides load(qidx, cell, pt) -> Data (point qidx = cell N^3 + pt
 // of the stored cell order) and apply(Data, val, grad), turning values and
@@ -245,7 +256,7 @@ struct QSolidT2F1 {
     d.s = stream_load(qd + 9 * nqp + q);
 #pragma unroll
     for (int f = 0; f < 6; ++f) d.T[f] = stream_load(qd + (10 + f) * nqp + q);
-    d.w = cell_jxw ? cell_jxw[cell] * wq[pt] : stream_load(qd + 16 * nqp + q);
+    d.w = cell_jxw ? as_constant(cell_jxw)[cell] * wq[pt] : stream_load(qd + 16 * nqp + q);
     return d;
   }
   __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
@@ -304,12 +315,12 @@ struct QSolidT2F1G {
 #pragma unroll
     for (int f = 0; f < 9; ++f) d.gu[f] = stream_load(qd + f * nqp + q);
     d.s = stream_load(qd + 9 * nqp + q);
-    d.w = cell_jxw[cell] * wq[pt];
+    d.w = as_constant(cell_jxw)[cell] * wq[pt];
     d.jac = cell_jac + (int64_t)cell * 18;
     return d;
   }
   __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
-    const Number *Jr = d.jac, *Jri = d.jac + 9;
+    const ConstantPtr<Number> Jr = as_constant(d.jac), Jri = as_constant(d.jac) + 9;
     Number ji[9];
     {
       Number Jc[3][3];
@@ -476,7 +487,7 @@ struct QHeat {
   __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
     if (cell_geo) {
-      const Number *g = cell_geo + (int64_t)cell * 10;
+      const ConstantPtr<Number> g = as_constant(cell_geo) + (int64_t)cell * 10;
 #pragma unroll
       for (int f = 0; f < 9; ++f) d.ji[f] = g[f];
       d.jxw = g[9] * wq[pt];
@@ -777,7 +788,10 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
       dold[c] = skip ? Number(0) : dst[node * NC + c];
     }
   }
-  typename QOp::Data qd = qop.load(qidx, active ? cell : 0, pt);
+  // one cell per workgroup: the cell index is the same on every lane (padding
+  // lanes included), which lets the per-cell tables come in as scalar loads
+  const int qcell = cpb == 1 ? __builtin_amdgcn_readfirstlane(cell < cell_end ? cell : 0) : (active ? cell : 0);
+  typename QOp::Data qd = qop.load(qidx, qcell, pt);
   __syncthreads();
   if constexpr (QOp::kNodalField) {
     // reference gradient of the cached nodal field (read plain: constrained
