@@ -21,8 +21,10 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def hierarchy(kind, lo, hi, coarse, n_levels, p, precisions=None, face_mask=1, amp=1e-3, noise=1e-5):
-    """Matching device and oracle hierarchies, linearised at interpolated u0."""
+def hierarchy(kind, lo, hi, coarse, n_levels, p, precisions=None, face_mask=1, amp=1e-3, noise=1e-5,
+              storage=mfgpu.T2_FOLDED):
+    """Matching device and oracle hierarchies, linearised at interpolated u0;
+    FP64 solid levels keep the given tensor2 point storage."""
     nc = 3 if kind == "solid" else 1
     cm = 7 if nc == 3 else 1
     precisions = precisions or [mfgpu.FP64] * n_levels
@@ -30,6 +32,9 @@ def hierarchy(kind, lo, hi, coarse, n_levels, p, precisions=None, face_mask=1, a
     oms = O.box_levels(lo, hi, coarse, n_levels, p, nc, face_mask, cm)
     if kind == "solid":
         gops = [mfgpu.NeoHookOperator(m, O.MU, O.NU, O.RHO, O.ALPHA1, precision=pr) for m, pr in zip(gms, precisions)]
+        for op, pr in zip(gops, precisions):
+            if pr == mfgpu.FP64:
+                op.set_point_storage(storage)
         oops = [O.Solid(m, caching=1) for m in oms]
     else:
         gops = [mfgpu.LaplaceOperator(m, 0.1, None, precision=pr) for m, pr in zip(gms, precisions)]
@@ -108,6 +113,27 @@ def test_mg_cg_small_config(cuda, coarse_kind):
     print(f"coarse_kind={coarse_kind}: iterations gpu {info.iterations} oracle {it_o}")
     assert abs(info.iterations - it_o) <= 1
     assert info.final_residual <= 1e-8 * np.linalg.norm(b)
+    assert rel(x, xo) < 1e-8
+    assert rel(x, xt) < 1e-6
+
+
+def test_mg_cg_gradient_storage(cuda):
+    """MG-CG with every level in the 10-word tensor2 storage: the oracle's
+    iteration count (+-1) and solution, as with the folded storage."""
+    gms, oms, gops, oops, trs = hierarchy("solid", [0, 0, 0], [1, 1, 1], [2, 2, 2], 3, 2, storage=mfgpu.T2_GRADIENT)
+    assert all(op.n_fields() == 10 for op in gops)
+    fine, A = oms[-1], oops[-1]
+    xt = O.fill_uniform(2, fine.n_dofs)
+    xt[fine.constrained()] = 0.0
+    b = A.vmult(xt)
+    xo, it_o, rel_o = O.mg_cg_solve(oms, oops, b, cheb_degree=4, smoothing_range=20.0, eig_cg_iterations=30,
+                                    coarse_kind=0, tol_rel=1e-8)
+    mg = mfgpu.Multigrid(gops, trs, cheb_degree=4, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0,
+                         coarse_tolerance=1e-3)
+    mg.setup()
+    x = torch.zeros(fine.n_dofs, dtype=torch.float64, device="cuda")
+    info = mfgpu.cg_solve(gops[-1], dev(b), x, mfgpu.PRECOND_MG, mg, tol_rel=1e-8)
+    assert abs(info.iterations - it_o) <= 1
     assert rel(x, xo) < 1e-8
     assert rel(x, xt) < 1e-6
 
