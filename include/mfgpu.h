@@ -391,7 +391,9 @@ typedef struct mfg_solid_problem_t *mfg_solid_problem;
  * children[l-1] connects level l-1 and l (see mfg_transfer_create).
  * level_precision: MFG_FP64 (plain FP64 V-cycle) or MFG_FP32 (FP32 V-cycle
  * and level caches; the coarse level, outer CG, residual and Newton update
- * stay FP64).  material->alpha_1 is ignored (1/(beta dt^2) is used). */
+ * stay FP64).  material->alpha_1 is ignored (1/(beta dt^2) is used).  The
+ * FP64 operators of formulation 1 on all-affine meshes keep the
+ * MFG_T2_GRADIENT point storage. */
 int mfg_solid_problem_create(int32_t n_levels, const mfg_level_desc *levels, const int32_t *const *children,
                              const mfg_solid_params *material, const mfg_newmark_params *newmark,
                              const mfg_mg_params *mg, int32_t level_precision, mfg_solid_problem *out);
