@@ -294,7 +294,8 @@ struct QSolidT2F1 {
 // the arithmetic of the cache kernel:
 //   J_cur = J_ref + Grad_xi u,   Jinv_cur = J_cur^{-1},
 //   F = I + Grad_xi u J_ref^{-1},   tau = mu F F^T - s I,
-// and JxW_ref = det J_ref w_q.  Fields: Grad_xi u [d][f] (9), s.
+// and JxW_ref = det J_ref w_q.  Fields: Grad_xi u [d][f] (9), s; stored in
+// pairs of fields (below).
 template <typename Number>
 struct QSolidT2F1G {
   static constexpr int NC = 3;
@@ -310,11 +311,23 @@ struct QSolidT2F1G {
     Number gu[9], s, w;
     const Number *jac;
   };
+  // fields are stored in pairs, qd[((f / 2) nqp + q) 2 + f % 2]: one 16-byte
+  // (FP64) load brings two fields of a point
+  typedef Number Pair __attribute__((ext_vector_type(2)));
   __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
+    const Pair *qp = reinterpret_cast<const Pair *>(qd);
 #pragma unroll
-    for (int f = 0; f < 9; ++f) d.gu[f] = stream_load(qd + f * nqp + q);
-    d.s = stream_load(qd + 9 * nqp + q);
+    for (int k = 0; k < 5; ++k) {
+      const Pair x = __builtin_nontemporal_load(qp + k * nqp + q);
+      if (k < 4) {
+        d.gu[2 * k] = x.x;
+        d.gu[2 * k + 1] = x.y;
+      } else {
+        d.gu[8] = x.x;
+        d.s = x.y;
+      }
+    }
     d.w = as_constant(cell_jxw)[cell] * wq[pt];
     d.jac = cell_jac + (int64_t)cell * 18;
     return d;

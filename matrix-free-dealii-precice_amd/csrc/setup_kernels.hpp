@@ -139,10 +139,10 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
   const int64_t n = a.nqp;
   const double wq = a.weights[T.i] * a.weights[T.j] * a.weights[T.k];
   const Number scalar = a.mu - Number(2) * a.lambda * log(J);
-  if (a.t2_gradient) {  // the 10-word tensor2 storage: Grad_xi u [d][f], s
+  if (a.t2_gradient) {  // the 10-word tensor2 storage: Grad_xi u [d][f], s, in pairs of fields
 #pragma unroll
-    for (int f = 0; f < 9; ++f) a.qd[f * n + qidx] = gr[f / 3][f % 3];
-    a.qd[9 * n + qidx] = scalar;
+    for (int f = 0; f < 9; ++f) a.qd[((f >> 1) * n + qidx) * 2 + (f & 1)] = gr[f / 3][f % 3];
+    a.qd[(4 * n + qidx) * 2 + 1] = scalar;
     return;
   }
   if (a.caching == MFG_CACHING_SCALAR_REFERENTIAL || a.caching == MFG_CACHING_TENSOR4_NS) {
