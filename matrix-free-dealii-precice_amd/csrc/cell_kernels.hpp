@@ -249,13 +249,24 @@ struct QSolidT2F1 {
   struct Data {
     Number ji[9], s, T[6], w;
   };
+  // fields 0..15 are stored in pairs, qd[((f / 2) nqp + q) 2 + f % 2], field
+  // 16 plainly after them: 8 16-byte (FP64) loads bring a point's 16 words
+  typedef Number Pair __attribute__((ext_vector_type(2)));
   __device__ Data load(int64_t q, int cell, int pt) const {
     Data d;
+    Number f16[16];
+    const Pair *qp = reinterpret_cast<const Pair *>(qd);
 #pragma unroll
-    for (int f = 0; f < 9; ++f) d.ji[f] = stream_load(qd + f * nqp + q);
-    d.s = stream_load(qd + 9 * nqp + q);
+    for (int k = 0; k < 8; ++k) {
+      const Pair x = __builtin_nontemporal_load(qp + k * nqp + q);
+      f16[2 * k] = x.x;
+      f16[2 * k + 1] = x.y;
+    }
 #pragma unroll
-    for (int f = 0; f < 6; ++f) d.T[f] = stream_load(qd + (10 + f) * nqp + q);
+    for (int f = 0; f < 9; ++f) d.ji[f] = f16[f];
+    d.s = f16[9];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) d.T[f] = f16[10 + f];
     d.w = cell_jxw ? as_constant(cell_jxw)[cell] * wq[pt] : stream_load(qd + 16 * nqp + q);
     return d;
   }

@@ -991,11 +991,11 @@ int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields) {
       if (op->precision == MFG_FP64)
         unpermute_fields_kernel<double><<<blocks, threads, 0, op->stream>>>(
             op->qd.as<double>(), (double *)out, op->perm_dev.as<int32_t>(), op->n_cells, n3, op->n_fields,
-            op->t2_gradient() ? 1 : 0);
+            op->kind == 0 && op->sp.caching == MFG_CACHING_TENSOR2 && op->sp.formulation == 1 ? 1 : 0);
       else
         unpermute_fields_kernel<float><<<blocks, threads, 0, op->stream>>>(
             op->qd.as<float>(), (float *)out, op->perm_dev.as<int32_t>(), op->n_cells, n3, op->n_fields,
-            op->t2_gradient() ? 1 : 0);
+            op->kind == 0 && op->sp.caching == MFG_CACHING_TENSOR2 && op->sp.formulation == 1 ? 1 : 0);
       MFG_HIP(hipGetLastError());
     }
   });

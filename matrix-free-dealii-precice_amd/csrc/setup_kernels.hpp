@@ -175,8 +175,11 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
   }
   inv3(Jc, Jci);
   const Number jxw = (Number)wq * fabs(det3(Jc));
+  // tensor2, formulation 1 keeps its first 16 fields in pairs (see QSolidT2F1)
+  const bool paired = a.formulation == 1 && a.caching == MFG_CACHING_TENSOR2;
+  auto at = [&](int f) -> int64_t { return paired && f < 16 ? ((f >> 1) * n + qidx) * 2 + (f & 1) : f * n + qidx; };
 #pragma unroll
-  for (int f = 0; f < 9; ++f) a.qd[f * n + qidx] = Jci[f / 3][f % 3];
+  for (int f = 0; f < 9; ++f) a.qd[at(f)] = Jci[f / 3][f % 3];
   if (a.formulation == 0) {
     // mf_nh_operator.h:419-440: J^{-1/3} and F
     a.qd[9 * n + qidx] = jxw;
@@ -218,13 +221,13 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
                       Jr[0][1] * (Jr[1][0] * Jr[2][2] - Jr[1][2] * Jr[2][0]) +
                       Jr[0][2] * (Jr[1][0] * Jr[2][1] - Jr[1][1] * Jr[2][0]);
   const Number jxw_ref = (Number)(wq * fabs(detr));
-  a.qd[9 * n + qidx] = jxw_ref * scalar;
+  a.qd[at(9)] = jxw_ref * scalar;
 #pragma unroll
   for (int s6 = 0; s6 < 6; ++s6) {
     const Number tau = a.mu * b[kSymI[s6]][kSymJ[s6]] - (kSymI[s6] == kSymJ[s6] ? scalar : Number(0));
-    a.qd[(10 + s6) * n + qidx] = jxw_ref * tau;
+    a.qd[at(10 + s6)] = jxw_ref * tau;
   }
-  if (a.jxw_ref_field) a.qd[16 * n + qidx] = jxw_ref;
+  if (a.jxw_ref_field) a.qd[at(16)] = jxw_ref;
 }
 
 // ---------------------------------------------------------- diagonal ---

@@ -31,7 +31,8 @@ __global__ void invert_kernel(const T *d, T *inv, int64_t n, int *flag) {
 
 // Per-(cell, point) field arrays from colour-major to input cell order:
 // out[f][perm[c]*n3 + q] = in[f][c*n3 + q]; paired: the input holds the
-// fields in pairs, in[((f/2) nqp + r) 2 + f%2]
+// fields in pairs, in[((f/2) nqp + r) 2 + f%2], an odd last field plainly
+// after them
 template <typename T>
 __global__ void unpermute_fields_kernel(const T *in, T *out, const int32_t *perm, int64_t n_cells, int n3,
                                         int n_fields, int paired) {
@@ -39,7 +40,8 @@ __global__ void unpermute_fields_kernel(const T *in, T *out, const int32_t *perm
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nqp * n_fields;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t f = i / nqp, r = i - f * nqp, c = r / n3, q = r - c * n3;
-    out[f * nqp + (int64_t)perm[c] * n3 + q] = in[paired ? ((f >> 1) * nqp + r) * 2 + (f & 1) : i];
+    const bool in_pair = paired && f < (n_fields & ~1);
+    out[f * nqp + (int64_t)perm[c] * n3 + q] = in[in_pair ? ((f >> 1) * nqp + r) * 2 + (f & 1) : i];
   }
 }
 
