@@ -773,6 +773,13 @@ template <typename Number, int NC>
 __device__ inline NodeValues<Number, NC> load_node(const Number *p) {
   return *reinterpret_cast<const NodeValues<Number, NC> *>(p);
 }
+template <typename Number, int NC>
+__device__ inline void store_node(Number *p, const Number (&v)[NC]) {
+  NodeValues<Number, NC> x;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) x.v[c] = v[c];
+  *reinterpret_cast<NodeValues<Number, NC> *>(p) = x;
+}
 
 template <int P, typename Number, class QOp, bool kToCells = false>
 __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ cell_nodes,
@@ -847,10 +854,7 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   Number out[NC];
   integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
   if constexpr (kToCells) {
-    if (active) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) dst[qidx * NC + c] = out[c];
-    }
+    if (active) store_node<Number, NC>(dst + qidx * NC, out);
     return;
   }
   if constexpr (kLateDst) {
@@ -860,9 +864,16 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
     for (int c = 0; c < NC; ++c) dold[c] = v.v[c];
   }
   if (active) {
+    Number r[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
-      if (!((entry >> (kConstraintShift + c)) & 1u)) dst[node * NC + c] = dold[c] + out[c];
+    for (int c = 0; c < NC; ++c) r[c] = dold[c] + out[c];
+    if (((entry >> kConstraintShift) & ((1u << NC) - 1u)) == 0u) {
+      store_node<Number, NC>(dst + node * NC, r);  // no constrained component: one wide store
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (!((entry >> (kConstraintShift + c)) & 1u)) dst[node * NC + c] = r[c];
+    }
   }
 }
 
