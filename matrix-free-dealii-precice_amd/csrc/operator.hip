@@ -16,12 +16,13 @@ using namespace mfg;
 
 namespace mfg {
 
-// cells per workgroup of the apply / cache kernels, per degree and
-// precision (Q4: one FP64 cell per workgroup measured 3-5 % faster than two
-// for the solid and the heat operator, two FP32 cells 3 % faster than one)
+// cells per workgroup of the apply / cache kernels, per degree (Q4: one cell
+// per workgroup measured 3-5 % faster than two for the solid and the heat
+// operator in FP64 and, once the per-cell tables became scalar loads, 2-5 %
+// in FP32)
 template <int P, typename Number>
 struct DefaultCPB {
-  static constexpr int value = P == 1 ? 8 : P == 2 ? 7 : P == 6 ? 1 : (P == 4 && sizeof(Number) == 8) ? 1 : 2;
+  static constexpr int value = P == 1 ? 8 : P == 2 ? 7 : P == 6 ? 1 : P == 4 ? 1 : 2;
 };
 
 // per-cell reference JxW of the tensor2 formulation-1 op, or null when it is
