@@ -118,20 +118,33 @@ static double dot(const vector<double> &a, const vector<double> &b) {
   return s;
 }
 
+// A level operator: a solid (kind 0) or heat (kind 1) oracle operator on a
+// mesh, or (kind 2) an explicit dense row-major matrix, which lets tests
+// check the smoother against closed forms on a small spectrum.
 struct LevelOp {
   int kind = 0;
   void *op = nullptr;
   Mesh *m = nullptr;
+  const double *dense = nullptr;
+  size_t dense_n = 0;
   vector<double> inv_diag;
-  size_t n() const { return (size_t)m->n_dofs; }
+  size_t n() const { return kind == 2 ? dense_n : (size_t)m->n_dofs; }
   void vmult(vector<double> &dst, const vector<double> &src) const {
     if (kind == 0) orc_solid_vmult(op, dst.data(), src.data());
-    else orc_heat_vmult(op, dst.data(), src.data());
+    else if (kind == 1) orc_heat_vmult(op, dst.data(), src.data());
+    else
+      for (size_t i = 0; i < dense_n; ++i) {
+        double s = 0;
+        for (size_t j = 0; j < dense_n; ++j) s += dense[i * dense_n + j] * src[j];
+        dst[i] = s;
+      }
   }
   void diagonal() {
     inv_diag.resize(n());
     if (kind == 0) orc_solid_diagonal(op, inv_diag.data());
-    else orc_heat_diagonal(op, inv_diag.data());
+    else if (kind == 1) orc_heat_diagonal(op, inv_diag.data());
+    else
+      for (size_t i = 0; i < dense_n; ++i) inv_diag[i] = dense[i * dense_n + i];
     for (auto &d : inv_diag) d = 1.0 / d;
   }
 };
@@ -171,10 +184,12 @@ static void tridiag_eigs(const vector<double> &dg, const vector<double> &off, do
 
 // Initial vector of the eigenvalue-estimation CG: 1 except every 11th
 // global index, constrained entries 0 (a fixed, partition-independent choice).
-static void eig_initial(const Mesh &m, vector<double> &v) {
-  v.resize((size_t)m.n_dofs);
-  for (int64_t i = 0; i < m.n_dofs; ++i) v[i] = (i % 11 == 0) ? 0.0 : 1.0;
-  for (int32_t c : m.constrained) v[c] = 0.0;
+static void eig_initial(const LevelOp &A, vector<double> &v) {
+  const size_t n = A.n();
+  v.resize(n);
+  for (size_t i = 0; i < n; ++i) v[i] = (i % 11 == 0) ? 0.0 : 1.0;
+  if (A.m)
+    for (int32_t c : A.m->constrained) v[c] = 0.0;
 }
 
 // Preconditioned CG that records the Lanczos coefficients (SolverCG with
@@ -222,7 +237,7 @@ struct Chebyshev {
   void setup(const LevelOp &op, int deg, double range, int eig_it) {
     A = &op;
     vector<double> v;
-    eig_initial(*op.m, v);
+    eig_initial(op, v);
     cg_eigen(op, v, eig_it, lmin, lmax);
     const double max_ev = 1.2 * lmax;
     const double alpha = range > 1.0 ? max_ev / range : std::min(0.9 * lmax, lmin);
@@ -256,7 +271,11 @@ struct Chebyshev {
     for (int k = 0; k < degree - 1; ++k) {
       A->vmult(t, x);
       const double rhokp = 1.0 / (2.0 * sigma - rhok);
+#ifdef ORC_PLANT_CHEB_F2
+      const double f1 = rhokp * rhok, f2 = rhokp / delta;  // planted mistake (mutation tests only)
+#else
       const double f1 = rhokp * rhok, f2 = 2.0 * rhokp / delta;
+#endif
       rhok = rhokp;
       for (size_t i = 0; i < n; ++i) {
         d[i] = f1 * d[i] + f2 * D[i] * (b[i] - t[i]);
@@ -315,7 +334,13 @@ struct MGSolver {
     mg_restrict_add(*meshes[l - 1], *meshes[l], tmp[l].data(), def[l - 1].data());
     v_cycle(l - 1);
     mg_prolongate_add(*meshes[l - 1], *meshes[l], sol[l - 1].data(), sol[l].data());
+#if defined(ORC_PLANT_POST_ZERO_GUESS)
+    cheb[l].vmult(sol[l], def[l]);  // planted mistake (mutation tests only): drops the coarse correction
+#elif defined(ORC_PLANT_NO_POST_SMOOTH)
+    // planted mistake (mutation tests only): no post-smoothing
+#else
     cheb[l].step(sol[l], def[l]);  // post-smoothing
+#endif
   }
   // PreconditionMG::vmult: copy_to_mg, V-cycle, copy_from_mg
   void precondition(vector<double> &dst, const vector<double> &src) const {
@@ -342,12 +367,33 @@ void orc_interpolate(void *cm, void *fm, const double *f, double *c) {
   mg_interpolate(*(Mesh *)cm, *(Mesh *)fm, f, c);
 }
 
+void orc_cheb_dense(int n, const double *Adense, int degree, double range, int eig_it, int zero_guess,
+                    const double *b, double *x, double *info) {
+  LevelOp A;
+  A.kind = 2;
+  A.dense = Adense;
+  A.dense_n = (size_t)n;
+  A.diagonal();
+  Chebyshev c;
+  c.setup(A, degree, range, eig_it);
+  vector<double> bb(b, b + n), xx(x, x + n);
+  c.run(xx, bb, zero_guess != 0);
+  std::copy(xx.begin(), xx.end(), x);
+  if (info) {
+    info[0] = c.lmax;
+    info[1] = c.lmin;
+    info[2] = c.theta;
+    info[3] = c.delta;
+    info[4] = c.degree;
+  }
+}
+
 void orc_cheb_eigenvalues(int kind, void *op, void *mesh, int n_it, double *lmax, double *lmin) {
   LevelOp A;
   A.kind = kind; A.op = op; A.m = (Mesh *)mesh;
   A.diagonal();
   vector<double> v;
-  eig_initial(*A.m, v);
+  eig_initial(A, v);
   cg_eigen(A, v, n_it, *lmin, *lmax);
 }
 

@@ -43,7 +43,7 @@ void orc_lagrange(int m, const double *z, int nx, const double *x, double *val,
  * deal.II colorize numbering (0:x=lo,1:x=hi,2:y=lo,3:y=hi,4:z=lo,5:z=hi);
  * comp_mask selects constrained components.
  * [pinned: closed-form node coordinates and constrained counts
- *  3*(ny*p+1)*(nz*p+1); tests/test_oracle_mesh.py] */
+ *  3*(ny*p+1)*(nz*p+1); tests/test_abi.py, tests/test_golden.py] */
 void *orc_mesh_create(const double *lo, const double *hi, const int *n, int p,
                       int nc, int face_mask, int comp_mask);
 void orc_mesh_destroy(void *m);
@@ -152,8 +152,11 @@ void orc_interpolate(void *coarse_mesh, void *fine_mesh, const double *f,
  * level-0 Chebyshev.  Returns iterations (or -1 on no convergence).
  * x is the initial guess on entry (zeroed by the caller in practice).
  * [pinned: convergence to the requested relative residual, rhs = A x_true
- *  recovers x_true; Chebyshev/eigenvalue bounds vs. dense eigen-solve on
- *  tiny levels] */
+ *  recovers x_true; Lanczos eigenvalue bounds vs. dense eigen-solve on tiny
+ *  levels; orc_mg_vmult with the Chebyshev coarse solver equals a V-cycle
+ *  composed independently in numpy from dense level matrices, dense transfer
+ *  and the closed-form Chebyshev polynomials, and is a symmetric operator;
+ *  tests/test_oracle_mg.py] */
 typedef struct {
   int cheb_degree;
   double smoothing_range;
@@ -183,6 +186,22 @@ int orc_newmark_step(int n_levels, void **meshes, void **ops,
 /* plain / Jacobi-preconditioned CG (omega scales the inverse diagonal) */
 int orc_cg_solve(int kind, void *op, int precond, double omega, const double *b,
                  double *x, double tol_rel, int max_iter, double *final_rel);
+/* Chebyshev smoother of the V-cycle levels on an explicit dense matrix A
+ * (row-major n x n, symmetric positive definite, no constrained entries):
+ * the same set-up as a level (eigenvalue estimate from eig_it CG steps on the
+ * documented start vector; interval [1.2 lmax / range, 1.2 lmax] for
+ * range > 1, or solver mode for degree <= 0 with the degree chosen from the
+ * estimate and range as the target reduction), then one application:
+ * zero_guess != 0: x = p(D^-1 A) D^-1 b (Chebyshev vmult); otherwise
+ * x += p(D^-1 A) D^-1 (b - A x) (Chebyshev step).
+ * info (optional, 5 values): lmax, lmin, theta, delta, degree.
+ * [pinned: the error propagator on every eigenvector of D^-1 A equals the
+ *  closed form T_k((theta-lambda)/delta)/T_k(theta/delta); the solver-mode
+ *  degree is the least k with T_k(theta/delta) > 1/range;
+ *  tests/test_oracle_mg.py, which also checks that planted mistakes in the
+ *  recurrence fail these pins] */
+void orc_cheb_dense(int n, const double *A, int degree, double range, int eig_it, int zero_guess,
+                    const double *b, double *x, double *info);
 /* Chebyshev smoother diagnostics on one level: returns the Lanczos max/min
  * eigenvalue estimates (of D^-1 A) from n_it CG iterations started on the
  * documented initial vector. */

@@ -26,7 +26,8 @@ class MGParams(ctypes.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # MFG_ORACLE_LIB: a copy built with a planted mistake (mutation tests only)
+        path = os.environ.get("MFG_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
         L = ctypes.CDLL(path)
@@ -74,6 +75,8 @@ def lib():
             "orc_cg_solve": (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, _dp, _dp,
                                             ctypes.c_double, ctypes.c_int, _dp]),
             "orc_cheb_eigenvalues": (None, [ctypes.c_int, _vp, _vp, ctypes.c_int, _dp, _dp]),
+            "orc_cheb_dense": (None, [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      _dp, _dp, _dp]),
             "orc_interface_traction": (None, [_vp, ctypes.c_int64, _ip, _dp, _dp, _dp]),
             "orc_face_values": (None, [_vp, ctypes.c_int64, _ip, _dp, _dp]),
             "orc_interface_flux": (None, [_vp, ctypes.c_int64, _ip, _dp, ctypes.c_double, _dp]),
@@ -381,6 +384,20 @@ def cheb_eigenvalues(op, n_it):
     lmax, lmin = ctypes.c_double(0), ctypes.c_double(0)
     lib().orc_cheb_eigenvalues(op.KIND, op.h, op.mesh.h, n_it, ctypes.byref(lmax), ctypes.byref(lmin))
     return lmax.value, lmin.value
+
+
+def cheb_dense(A, b, degree, smoothing_range, eig_it, x0=None):
+    """One Chebyshev application on the dense matrix A: x = p(D^-1 A) D^-1 b
+    from a zero guess (x0 None), else x0 + p(D^-1 A) D^-1 (b - A x0).
+    Returns (x, dict(lmax, lmin, theta, delta, degree))."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    info = np.zeros(5)
+    lib().orc_cheb_dense(n, dptr(A), degree, smoothing_range, eig_it, 1 if x0 is None else 0, dptr(b), dptr(x),
+                         dptr(info))
+    return x, dict(lmax=info[0], lmin=info[1], theta=info[2], delta=info[3], degree=int(info[4]))
 
 
 def box_levels(lo, hi, coarse_n, n_levels, p, nc, face_mask=1, comp_mask=7):
