@@ -201,6 +201,10 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
   MFG_REQUIRE(n_levels == 1 || children, MFG_ERR_ARGUMENT, "children tables required");
   MFG_REQUIRE(level_precision == MFG_FP64 || level_precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
   MFG_REQUIRE(newmark.dt > 0 && newmark.beta > 0, MFG_ERR_ARGUMENT, "bad Newmark parameters");
+  MFG_REQUIRE(material.mu > 0 && material.nu > -1 && material.nu < 0.5 && material.rho >= 0, MFG_ERR_ARGUMENT,
+              "bad material");
+  MFG_REQUIRE(material.formulation == 0 || material.formulation == 1, MFG_ERR_ARGUMENT,
+              "Unknown material formulation");
   auto p = std::make_unique<SolidProblem>();
   p->L = n_levels;
   p->level_precision = level_precision;

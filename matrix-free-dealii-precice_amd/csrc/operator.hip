@@ -274,6 +274,8 @@ static void diagonal_t(OperatorData &op) {
 void OperatorData::apply(const void *src, void *dst, bool add) const {
   MFG_REQUIRE(ready, MFG_ERR_STATE, "operator used before cache()");
   MFG_REQUIRE(src && dst, MFG_ERR_ARGUMENT, "null vector");
+  // the scatter reads dst while other workgroups still gather src
+  MFG_REQUIRE(src != dst, MFG_ERR_ARGUMENT, "dst and src must be different vectors");
   // vmult needs no zero fill: the first colour touching a node stores
   timing_begin();
   if (precision == MFG_FP64) apply_t<double>(*this, src, dst, add ? 2 : 1);
@@ -642,6 +644,11 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
     }
     for (int x = 0; x < 24; ++x) vert[(size_t)c * 24 + x] = d.cell_vertices[o * 24 + x];
   }
+  // vmult writes every dof from the cells that reference its node (the first
+  // colour stores, the GATHER sum starts from the first contribution): a node
+  // no cell references would keep stale dst, so such a description is refused
+  for (int64_t node = 0; node < n_nodes; ++node)
+    MFG_REQUIRE(touched[node], MFG_ERR_ARGUMENT, "node referenced by no cell");
   // affine cells: the mixed terms of the trilinear map vanish (vertex v at
   // (v & 1, v >> 1 & 1, v >> 2 & 1)); then det J_ref is one number per cell
   std::vector<double> det_ref((size_t)n_cells), jac((size_t)n_cells * 18);

@@ -71,13 +71,16 @@ struct Session {
     built = true;
   }
 
+  // prob becomes non-null only once the problem and its state are on the
+  // device, so a failure here leaves the session non-resident
   void make_resident(hipStream_t s) {
-    prob = make_solid_problem((int)descs.size(), descs.data(), child_ptrs.empty() ? nullptr : child_ptrs.data(),
-                              spec.material, spec.newmark, spec.mg, spec.level_precision, s);
-    MFG_HIP(hipMemcpyAsync(prob->u.ptr, hu.data(), n_dofs * 8, hipMemcpyHostToDevice, s));
-    MFG_HIP(hipMemcpyAsync(prob->v_old.ptr, hv.data(), n_dofs * 8, hipMemcpyHostToDevice, s));
-    MFG_HIP(hipMemcpyAsync(prob->a_old.ptr, ha.data(), n_dofs * 8, hipMemcpyHostToDevice, s));
+    auto P = make_solid_problem((int)descs.size(), descs.data(), child_ptrs.empty() ? nullptr : child_ptrs.data(),
+                                spec.material, spec.newmark, spec.mg, spec.level_precision, s);
+    MFG_HIP(hipMemcpyAsync(P->u.ptr, hu.data(), n_dofs * 8, hipMemcpyHostToDevice, s));
+    MFG_HIP(hipMemcpyAsync(P->v_old.ptr, hv.data(), n_dofs * 8, hipMemcpyHostToDevice, s));
+    MFG_HIP(hipMemcpyAsync(P->a_old.ptr, ha.data(), n_dofs * 8, hipMemcpyHostToDevice, s));
     MFG_HIP(hipStreamSynchronize(s));
+    prob = std::move(P);
   }
 
   void save_state(hipStream_t s) {
@@ -163,6 +166,7 @@ struct Server {
     for (;;) {
       std::vector<Session *> evict;
       int r = -1;
+      bool admitted = false;  // pick() reserved the session's bytes
       {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
@@ -172,6 +176,7 @@ struct Server {
           cv.wait_for(lk, std::chrono::milliseconds(1));
         }
         if (r < 0) break;
+        admitted = !sessions[requests[r].session]->prob;
       }
       Session &S = *sessions[requests[r].session];
       mfg_request_result &res = results[r];
@@ -201,6 +206,20 @@ struct Server {
       res.end_s = now();
       {
         std::lock_guard<std::mutex> g(mu);
+        // pick() counted the evictions and the admission as done; undo what a
+        // failure above left undone, so the budget keeps matching the
+        // sessions that are actually resident
+        for (Session *v : evict)
+          if (v->prob) {
+            resident_bytes += v->bytes;
+            ++resident;
+            --evictions;
+          }
+        if (admitted && !S.prob) {
+          resident_bytes -= S.bytes;
+          --resident;
+          --admissions;
+        }
         for (Session *v : evict) v->busy = false;
         S.busy = false;
         S.last_used = res.end_s;
@@ -265,6 +284,9 @@ int mfg_serve(const mfg_session_spec *sessions, int32_t n_sessions, const mfg_re
     Server srv;
     srv.prm = *params;
     for (int i = 0; i < n_sessions; ++i) {
+      const mfg_newmark_params &nm = sessions[i].newmark;
+      MFG_REQUIRE(nm.dt > 0 && nm.beta > 0 && nm.gamma >= 0 && nm.max_newton >= 1 && nm.max_lin >= 1,
+                  MFG_ERR_ARGUMENT, "bad Newmark parameters");
       auto s = std::make_unique<Session>();
       s->spec = sessions[i];
       s->build_hierarchy();

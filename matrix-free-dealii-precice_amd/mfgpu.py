@@ -156,6 +156,27 @@ def _dtype_of(precision):
     return torch.float64 if precision == FP64 else torch.float32
 
 
+def _check_vec(t, n, dtype, name="vector"):
+    """Reject, before any launch, a vector that the kernels would index out of
+    bounds or misread: it must be a contiguous device tensor of n entries of
+    the operator's precision."""
+    if not hasattr(t, "data_ptr") or not t.is_cuda:
+        raise ValueError(f"{name}: expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype} does not match the operator's {dtype}")
+    if t.numel() != n:
+        raise ValueError(f"{name}: {t.numel()} entries, the operator has {n} rows")
+
+
+def _check_pair(dst, src, n, dtype):
+    _check_vec(dst, n, dtype, "dst")
+    _check_vec(src, n, dtype, "src")
+    if dst.data_ptr() == src.data_ptr():
+        raise ValueError("dst and src must be different vectors")
+
+
 class BoxMesh:
     """Structured box mesh level (cells, nodes, Dirichlet dofs) built on the host."""
 
@@ -252,15 +273,19 @@ class _Operator:
         check(lib().mfg_operator_set_stream(self.h, ctypes.c_void_p(int(stream.cuda_stream))))
 
     def vmult(self, dst, src):
+        _check_pair(dst, src, self.m(), self.dtype)
         check(lib().mfg_operator_vmult(self.h, _ptr(dst), _ptr(src)))
 
     def vmult_add(self, dst, src):
+        _check_pair(dst, src, self.m(), self.dtype)
         check(lib().mfg_operator_vmult_add(self.h, _ptr(dst), _ptr(src)))
 
     def Tvmult(self, dst, src):
+        _check_pair(dst, src, self.m(), self.dtype)
         check(lib().mfg_operator_Tvmult(self.h, _ptr(dst), _ptr(src)))
 
     def Tvmult_add(self, dst, src):
+        _check_pair(dst, src, self.m(), self.dtype)
         check(lib().mfg_operator_Tvmult_add(self.h, _ptr(dst), _ptr(src)))
 
     def compute_diagonal(self):
@@ -284,6 +309,7 @@ class _Operator:
         return v.value
 
     def precondition_Jacobi(self, dst, src, omega):
+        _check_pair(dst, src, self.m(), self.dtype)
         check(lib().mfg_operator_precondition_jacobi(self.h, _ptr(dst), _ptr(src), omega))
 
     def memory_consumption(self):
@@ -497,6 +523,9 @@ class Multigrid:
 
     def vmult(self, dst, src):
         import torch
+        if src.dtype not in (torch.float64, torch.float32):
+            raise ValueError("src: expected float64 or float32")
+        _check_pair(dst, src, self._keep[0][-1].m(), src.dtype)
         prec = FP64 if src.dtype == torch.float64 else FP32
         check(lib().mfg_mg_vmult(self.h, _ptr(dst), _ptr(src), prec))
 
@@ -508,6 +537,10 @@ class Multigrid:
 
 def cg_solve(A, b, x, preconditioner=PRECOND_NONE, mg=None, omega=1.0, tol_rel=1e-8, max_iter=10000):
     """SolverCG::solve; returns SolveInfo.  Raises MfgError(4) on no convergence."""
+    _check_vec(b, A.m(), A.dtype, "b")
+    _check_vec(x, A.m(), A.dtype, "x")
+    if b.data_ptr() == x.data_ptr():
+        raise ValueError("b and x must be different vectors")
     info = SolveInfo()
     st = lib().mfg_cg_solve(A.h, preconditioner, None if mg is None else mg.h, omega, _ptr(b), _ptr(x), tol_rel,
                             max_iter, ctypes.byref(info))

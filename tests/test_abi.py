@@ -64,3 +64,18 @@ def test_constrained_count_closed_forms():
     m = mfgpu.BoxMesh([0, 0, 0], [4, 1, 1], [128, 32, 32], 4, 3, 1, 7)
     assert m.n_constrained == 3 * 129 * 129 == 49923
     assert m.n_dofs == 513 * 129 * 129 * 3 == 25610499
+
+
+def test_node_referenced_by_no_cell_is_refused():
+    """A level whose node list has a node no cell references would leave that
+    node's dst entries unwritten by vmult; the layout builder refuses it before
+    any device work."""
+    import ctypes
+    m = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 1, 3, 1, 7)
+    d = mfgpu.LevelDesc()
+    ctypes.pointer(d)[0] = m.desc
+    d.n_nodes = m.n_nodes + 1
+    prm = mfgpu.SolidParams(O.MU, O.NU, O.RHO, O.ALPHA1, 1, 1)
+    h = ctypes.c_void_p()
+    st = mfgpu.lib().mfg_solid_create(ctypes.byref(d), ctypes.byref(prm), mfgpu.FP64, ctypes.byref(h))
+    assert st == 1 and b"no cell" in mfgpu.lib().mfg_last_error()

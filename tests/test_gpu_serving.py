@@ -67,3 +67,28 @@ def test_session_larger_than_budget_is_rejected(cuda):
     sp = specs()[:1]
     with pytest.raises(mfgpu.MfgError):
         mfgpu.serve(sp, [(0, 0.0)], hbm_budget_bytes=1024)
+
+
+@pytest.mark.timeout(120)
+def test_failed_admission_releases_its_budget(cuda):
+    """A session whose problem cannot be built (bad material, refused when it
+    is made resident) fails its requests; the bytes reserved for it are
+    released, so the other session, too large to share the budget with it,
+    is still admitted and served, and the call returns an error instead of
+    waiting for budget that never frees."""
+    good = specs()[:1]
+    bad = mfgpu.session_spec([0, 0, 0], [4.0, 1.0, 1.0], [8, 2, 2], 2, -1.0, O.NU, O.RHO, seed=9,
+                             level_precision=mfgpu.FP32)
+    _, st, _ = mfgpu.serve(good, [(0, 0.0)], max_concurrent=1, real_time=False)
+    budget = int(st.peak_bytes * 1.5)  # room for one of the two sessions at a time
+    reqs = [(1, 0.0), (1, 0.0), (0, 0.0), (0, 0.0)]
+    with pytest.raises(mfgpu.MfgError) as e:
+        mfgpu.serve(good + [bad], reqs, max_concurrent=2, hbm_budget_bytes=budget, real_time=False)
+    assert "bad material" in str(e.value)
+
+
+def test_bad_newmark_parameters_rejected_before_serving(cuda):
+    bad = mfgpu.session_spec([0, 0, 0], [4.0, 1.0, 1.0], [8, 2, 2], 2, O.MU, O.NU, O.RHO, dt=0.0)
+    with pytest.raises(mfgpu.MfgError) as e:
+        mfgpu.serve([bad], [(0, 0.0), (0, 0.0)])
+    assert "Newmark" in str(e.value)
