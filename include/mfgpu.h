@@ -87,6 +87,12 @@ typedef struct mfg_level_desc {
 typedef struct mfg_mesh_t *mfg_mesh;
 int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n_cells[3], int32_t degree,
                         int32_t n_components, int32_t face_mask, int32_t comp_mask, mfg_mesh *out);
+/* The same with a component mask per boundary face (colorize numbering as
+ * above): the per-boundary-id ComponentMasks of make_constraints
+ * (mf_elasticity.h:1918-1974), e.g. a flap clamped on x = lo and held in z on
+ * the z faces (turek_hron.h:45-58: face_comp_mask = {7, 0, 0, 0, 4, 4}). */
+int mfg_box_mesh_create_bc(const double lo[3], const double hi[3], const int32_t n_cells[3], int32_t degree,
+                           int32_t n_components, const int32_t face_comp_mask[6], mfg_mesh *out);
 int mfg_box_mesh_destroy(mfg_mesh mesh);
 /* Fills desc with pointers into the mesh's host arrays (valid while mesh lives). */
 int mfg_box_mesh_describe(mfg_mesh mesh, mfg_level_desc *desc);
@@ -378,6 +384,12 @@ typedef struct mfg_newmark_params {
   double tol_f, tol_u, tol_f_abs, tol_u_abs;
   double tol_lin;   /* linear solve: ||r|| <= tol_lin ||R|| */
   int32_t max_lin;  /* CG iteration cap per linear solve */
+  /* Condition estimate before every linear solve: this many plain CG
+   * iterations on the fine operator (stopping early at ||r|| <= tol_lin ||R||),
+   * their Lanczos condition number reported and the iterate discarded
+   * (solve_linear_system, mf_elasticity.h:1998-2013; the reference's default
+   * is on with 20 iterations, parameter_handling.h:183-184).  0 skips it. */
+  int32_t cond_cg_iterations;
 } mfg_newmark_params;
 typedef struct mfg_step_info {
   int32_t newton_iterations;
@@ -385,15 +397,17 @@ typedef struct mfg_step_info {
   int32_t converged;
   double residual_0, residual_final, update_final;
   double seconds;
+  double condition_estimate;  /* of the last linear solve (1 when not estimated) */
 } mfg_step_info;
 typedef struct mfg_solid_problem_t *mfg_solid_problem;
 /* Level hierarchy coarse..fine (levels[n_levels-1] is the active mesh);
  * children[l-1] connects level l-1 and l (see mfg_transfer_create).
  * level_precision: MFG_FP64 (plain FP64 V-cycle) or MFG_FP32 (FP32 V-cycle
  * and level caches; the coarse level, outer CG, residual and Newton update
- * stay FP64).  material->alpha_1 is ignored (1/(beta dt^2) is used).  The
- * FP64 operators of formulation 1 on all-affine meshes keep the
- * MFG_T2_GRADIENT point storage. */
+ * stay FP64).  material->alpha_1 is ignored (1/(beta dt^2) is used);
+ * material->caching selects the operators' caching (initialize's `caching`
+ * argument, mf_nh_operator.h:115-119).  The FP64 tensor2 operators of
+ * formulation 1 on all-affine meshes keep the MFG_T2_GRADIENT point storage. */
 int mfg_solid_problem_create(int32_t n_levels, const mfg_level_desc *levels, const int32_t *const *children,
                              const mfg_solid_params *material, const mfg_newmark_params *newmark,
                              const mfg_mg_params *mg, int32_t level_precision, mfg_solid_problem *out);

@@ -21,42 +21,72 @@ namespace mfg {
 static int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 8192); }
 
 Reducer::Reducer() {
-  partials.allocate(kRedBlocks * sizeof(double));
-  result.allocate(8 * sizeof(double));
+  own.allocate((kRedBlocks + kRedScalars) * sizeof(double));
+  partials = own.as<double>();
+  scal = partials + kRedBlocks;
 }
 
+Reducer::Reducer(double *p, double *sc) : partials(p), scal(sc) {}
+
 template <typename T>
-double Reducer::dot_t(const void *a, const void *b, int64_t n, hipStream_t s) {
-  dot_partial_kernel<T><<<kRedBlocks, kRedThreads, 0, s>>>((const T *)a, (const T *)b, n, partials.as<double>());
-  sum_partials_kernel<<<1, kRedThreads, 0, s>>>(partials.as<double>(), kRedBlocks, result.as<double>());
+static void dot_partials(const void *a, const void *b, int64_t n, double *partials, hipStream_t s) {
+  dot_partial_kernel<T><<<kRedBlocks, kRedThreads, 0, s>>>((const T *)a, (const T *)b, n, partials);
+}
+
+void Reducer::epilogue(int mode, int slot, double tol, double rel, bool record, hipStream_t s) {
+  sum_partials_epilogue_kernel<<<1, kRedThreads, 0, s>>>(partials, kRedBlocks, scal, mode, slot, tol, rel,
+                                                         record ? hist_buf.as<double>() : nullptr, hist_cap);
   MFG_HIP(hipGetLastError());
+}
+
+double Reducer::dot(int prec, const void *a, const void *b, int64_t n, hipStream_t s) {
+  if (prec == MFG_FP64) dot_partials<double>(a, b, n, partials, s);
+  else dot_partials<float>(a, b, n, partials, s);
+  epilogue(kRedSet, kSlotHost, 0, 0, false, s);
   double h = 0;
-  MFG_HIP(hipMemcpyAsync(&h, result.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+  MFG_HIP(hipMemcpyAsync(&h, scal + kSlotHost, sizeof(double), hipMemcpyDeviceToHost, s));
   MFG_HIP(hipStreamSynchronize(s));
   return h;
 }
 
-double Reducer::dot(int prec, const void *a, const void *b, int64_t n, hipStream_t s) {
-  return prec == MFG_FP64 ? dot_t<double>(a, b, n, s) : dot_t<float>(a, b, n, s);
+void Reducer::dot_dev(int prec, const void *a, const void *b, int64_t n, int mode, int slot, hipStream_t s,
+                      double tol, double rel, bool record) {
+  if (prec == MFG_FP64) dot_partials<double>(a, b, n, partials, s);
+  else dot_partials<float>(a, b, n, partials, s);
+  epilogue(mode, slot, tol, rel, record, s);
 }
 
-template <typename T>
-double Reducer::cg_update_t(void *x, void *r, const void *p, const void *h, double alpha, int64_t n,
+void Reducer::cg_update_dev(int prec, void *x, void *r, const void *p, const void *h, int64_t n, int mode,
                             hipStream_t s) {
-  cg_update_kernel<T><<<kRedBlocks, kRedThreads, 0, s>>>((T *)x, (T *)r, (const T *)p, (const T *)h, alpha, n,
-                                                         partials.as<double>());
-  sum_partials_kernel<<<1, kRedThreads, 0, s>>>(partials.as<double>(), kRedBlocks, result.as<double>());
-  MFG_HIP(hipGetLastError());
-  double v = 0;
-  MFG_HIP(hipMemcpyAsync(&v, result.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
-  MFG_HIP(hipStreamSynchronize(s));
-  return v;
+  if (prec == MFG_FP64)
+    cg_update_dev_kernel<double><<<kRedBlocks, kRedThreads, 0, s>>>((double *)x, (double *)r, (const double *)p,
+                                                                    (const double *)h, scal, n, partials);
+  else
+    cg_update_dev_kernel<float><<<kRedBlocks, kRedThreads, 0, s>>>((float *)x, (float *)r, (const float *)p,
+                                                                   (const float *)h, scal, n, partials);
+  epilogue(mode, kSlotRR, 0, 0, false, s);
 }
 
-double Reducer::cg_update(int prec, void *x, void *r, const void *p, const void *h, double alpha, int64_t n,
-                          hipStream_t s) {
-  return prec == MFG_FP64 ? cg_update_t<double>(x, r, p, h, alpha, n, s)
-                          : cg_update_t<float>(x, r, p, h, alpha, n, s);
+void Reducer::read(double *out, hipStream_t s) {
+  MFG_HIP(hipMemcpyAsync(out, scal, kRedScalars * sizeof(double), hipMemcpyDeviceToHost, s));
+  MFG_HIP(hipStreamSynchronize(s));
+}
+
+void Reducer::ensure_hist(int cap) {
+  if (cap <= hist_cap) return;
+  hist_buf.allocate((size_t)2 * cap * sizeof(double));
+  hist_cap = cap;
+}
+
+void Reducer::read_hist(const double *sc, std::vector<double> &alphas, std::vector<double> &betas, hipStream_t s) {
+  const int na = (int)sc[kSlotNAlpha], nb = (int)sc[kSlotNBeta];
+  alphas.assign(na, 0.0);
+  betas.assign(nb, 0.0);
+  if (na) MFG_HIP(hipMemcpyAsync(alphas.data(), hist_buf.ptr, na * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (nb)
+    MFG_HIP(hipMemcpyAsync(betas.data(), hist_buf.as<double>() + hist_cap, nb * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+  MFG_HIP(hipStreamSynchronize(s));
 }
 
 void vec_fill(int prec, void *x, int64_t n, double v, hipStream_t s) {
@@ -92,6 +122,11 @@ static void xpby_t(void *p, const void *d, double beta, int64_t n, hipStream_t s
 void vec_xpby(int prec, void *p, const void *d, double beta, int64_t n, hipStream_t s) {
   if (prec == MFG_FP64) xpby_t<double>(p, d, beta, n, s);
   else xpby_t<float>(p, d, beta, n, s);
+  MFG_HIP(hipGetLastError());
+}
+void Reducer::xpby_dev(int prec, void *p, const void *z, int64_t n, hipStream_t s) {
+  if (prec == MFG_FP64) xpby_dev_kernel<double><<<grid_for(n), 256, 0, s>>>((double *)p, (const double *)z, scal, n);
+  else xpby_dev_kernel<float><<<grid_for(n), 256, 0, s>>>((float *)p, (const float *)z, scal, n);
   MFG_HIP(hipGetLastError());
 }
 template <typename T>
@@ -322,25 +357,32 @@ void MGLevel::estimate_eigenvalues(int n_it, Reducer &red) {
   vec_fill(pr, sol.ptr, n, 0.0, s);
   vec_convert(pr, cg_r.ptr, pr, tmp.ptr, n, s);
   vec_jacobi(pr, cg_z.ptr, A.inv_diag.ptr, cg_r.ptr, 1.0, n, s);
-  double gh = red.dot(pr, cg_r.ptr, cg_z.ptr, n, s);
+  // The iterations run without host round trips: the step lengths live in
+  // the reducer's scalar block and history; the stop flag (breakdown, or
+  // ||r|| <= 1e-15 ||r_0||) turns the remaining iterations into no-ops.  Long
+  // runs (the coarse level's m() iterations) look at the flag every 16
+  // iterations to skip the no-ops.
+  red.ensure_hist(std::max(n_it, 1));
+  red.dot_dev(pr, cg_r.ptr, cg_z.ptr, n, kRedStart, 0, s);
   vec_convert(pr, cg_p.ptr, pr, cg_z.ptr, n, s);
-  const double res0 = std::sqrt(red.dot(pr, cg_r.ptr, cg_r.ptr, n, s));
-  std::vector<double> alphas, betas;
-  for (int it = 0; it < n_it && res0 > 0; ++it) {
+  red.dot_dev(pr, cg_r.ptr, cg_r.ptr, n, kRedRes0, 0, s, 0.0, 1e-15);
+  double sc[kRedScalars];
+  for (int it = 0; it < n_it; ++it) {
     op_vmult(A, cg_h.ptr, cg_p.ptr);
-    const double ph = red.dot(pr, cg_p.ptr, cg_h.ptr, n, s);
-    if (!(ph > 0)) break;
-    const double alpha = gh / ph;
-    const double rr = red.cg_update(pr, sol.ptr, cg_r.ptr, cg_p.ptr, cg_h.ptr, alpha, n, s);
-    alphas.push_back(alpha);
-    if (std::sqrt(rr) <= 1e-15 * res0 || it + 1 == n_it) break;
+    red.dot_dev(pr, cg_p.ptr, cg_h.ptr, n, kRedAlpha, 0, s, 0, 0, true);
+    red.cg_update_dev(pr, sol.ptr, cg_r.ptr, cg_p.ptr, cg_h.ptr, n, kRedRRStop, s);
+    if (it + 1 == n_it) break;
     vec_jacobi(pr, cg_z.ptr, A.inv_diag.ptr, cg_r.ptr, 1.0, n, s);
-    const double ghn = red.dot(pr, cg_r.ptr, cg_z.ptr, n, s);
-    const double beta = ghn / gh;
-    gh = ghn;
-    betas.push_back(beta);
-    vec_xpby(pr, cg_p.ptr, cg_z.ptr, beta, n, s);
+    red.dot_dev(pr, cg_r.ptr, cg_z.ptr, n, kRedBeta, 0, s, 0, 0, true);
+    red.xpby_dev(pr, cg_p.ptr, cg_z.ptr, n, s);
+    if (n_it > 32 && (it + 1) % 16 == 0) {
+      red.read(sc, s);
+      if (sc[kSlotStop] != 0) break;
+    }
   }
+  red.read(sc, s);
+  std::vector<double> alphas, betas;
+  red.read_hist(sc, alphas, betas, s);
   if (alphas.size() > betas.size() + 1) alphas.resize(betas.size() + 1);
   lanczos_estimate(alphas, betas, lmin, lmax);
 }
@@ -441,23 +483,26 @@ void MGData::coarse_solve() {
   hipStream_t s = A.stream;
   vec_fill(pr, L.sol.ptr, n, 0.0, s);
   vec_convert(pr, L.cg_r.ptr, pr, L.def.ptr, n, s);
-  const double res0 = std::sqrt(red.dot(pr, L.cg_r.ptr, L.cg_r.ptr, n, s));
+  // step lengths stay on the device; one read-back per iteration (||r||)
+  double sc[kRedScalars];
+  red.dot_dev(pr, L.cg_r.ptr, L.cg_r.ptr, n, kRedSet, kSlotRR, s);
+  red.read(sc, s);
+  const double res0 = std::sqrt(sc[kSlotRR]);
   const double tol = std::max(1e-10, prm.coarse_tolerance * res0);
   if (res0 <= tol) return;
   L.chebyshev(L.cg_z.ptr, L.cg_r.ptr, true);
-  double gh = red.dot(pr, L.cg_r.ptr, L.cg_z.ptr, n, s);
+  red.dot_dev(pr, L.cg_r.ptr, L.cg_z.ptr, n, kRedStart, 0, s);
   vec_convert(pr, L.cg_p.ptr, pr, L.cg_z.ptr, n, s);
   const int maxit = std::max<int>((int)n, 100);
   for (int it = 0; it < maxit; ++it) {
     op_vmult(A, L.cg_h.ptr, L.cg_p.ptr);
-    const double alpha = gh / red.dot(pr, L.cg_p.ptr, L.cg_h.ptr, n, s);
-    const double rr = red.cg_update(pr, L.sol.ptr, L.cg_r.ptr, L.cg_p.ptr, L.cg_h.ptr, alpha, n, s);
-    if (std::sqrt(rr) <= tol) return;
+    red.dot_dev(pr, L.cg_p.ptr, L.cg_h.ptr, n, kRedAlpha, 0, s);
+    red.cg_update_dev(pr, L.sol.ptr, L.cg_r.ptr, L.cg_p.ptr, L.cg_h.ptr, n, kRedSet, s);
+    red.read(sc, s);
+    if (sc[kSlotBreakdown] != 0 || std::sqrt(sc[kSlotRR]) <= tol) return;
     L.chebyshev(L.cg_z.ptr, L.cg_r.ptr, true);
-    const double ghn = red.dot(pr, L.cg_r.ptr, L.cg_z.ptr, n, s);
-    const double beta = ghn / gh;
-    gh = ghn;
-    vec_xpby(pr, L.cg_p.ptr, L.cg_z.ptr, beta, n, s);
+    red.dot_dev(pr, L.cg_r.ptr, L.cg_z.ptr, n, kRedBeta, 0, s);
+    red.xpby_dev(pr, L.cg_p.ptr, L.cg_z.ptr, n, s);
   }
 }
 
@@ -591,11 +636,10 @@ void MGData::vmult(void *dst, const void *src, int vec_prec) {
 
 // ------------------------------------------------------------------ CG ---
 void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, const void *b, void *x, double tol_rel,
-              int max_iter, mfg_solve_info *info, CGWorkspace &ws, Reducer &red) {
+              int max_iter, mfg_solve_info *info, const CGVectors &v, Reducer &red) {
   const int64_t n = A.n_dofs;
   const int pr = A.precision;
   hipStream_t s = A.stream;
-  ws.ensure(n * A.elem_size());
   if (precond == MFG_PRECOND_JACOBI) MFG_REQUIRE(A.has_diagonal, MFG_ERR_STATE, "compute_diagonal() not called");
   if (precond == MFG_PRECOND_MG) MFG_REQUIRE(mg && mg->ready, MFG_ERR_STATE, "multigrid not set up");
   auto apply_precond = [&](void *z, const void *r) {
@@ -603,35 +647,49 @@ void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, cons
     else if (precond == MFG_PRECOND_JACOBI) vec_jacobi(pr, z, A.inv_diag.ptr, r, omega, n, s);
     else mg->vmult(z, r, pr);
   };
+  // The step lengths are computed on the device from reductions that stay in
+  // the reducer's scalar block; the host reads the block once per iteration
+  // for ||r|| (and the coefficients it keeps for the condition estimate).
+  double sc[kRedScalars];
   // r = b - A x
-  op_vmult(A, ws.h.ptr, x);
-  vec_add_scaled(pr, ws.r.ptr, b, -1.0, ws.h.ptr, n, s);
-  const double bnorm = std::sqrt(red.dot(pr, b, b, n, s));
+  op_vmult(A, v.h, x);
+  vec_add_scaled(pr, v.r, b, -1.0, v.h, n, s);
+  red.dot_dev(pr, b, b, n, kRedSet, kSlotB2, s);
+  red.dot_dev(pr, v.r, v.r, n, kRedSet, kSlotRR, s);
+  red.read(sc, s);
+  const double bnorm = std::sqrt(sc[kSlotB2]);
   const double tol = tol_rel * bnorm;
-  double res = std::sqrt(red.dot(pr, ws.r.ptr, ws.r.ptr, n, s));
+  double res = std::sqrt(sc[kSlotRR]);
   info->initial_residual = res;
   info->iterations = 0;
   info->condition_estimate = 1.0;
   std::vector<double> alphas, betas;
-  if (res > tol) {
-    apply_precond(ws.z.ptr, ws.r.ptr);
-    double gh = red.dot(pr, ws.r.ptr, ws.z.ptr, n, s);
-    vec_convert(pr, ws.p.ptr, pr, ws.z.ptr, n, s);
+  bool converged = res <= tol;
+  if (!converged) {
+    apply_precond(v.z, v.r);
+    red.dot_dev(pr, v.r, v.z, n, kRedStart, 0, s);
+    vec_convert(pr, v.p, pr, v.z, n, s);
     for (int it = 1; it <= max_iter; ++it) {
-      op_vmult(A, ws.h.ptr, ws.p.ptr);
-      const double ph = red.dot(pr, ws.p.ptr, ws.h.ptr, n, s);
-      MFG_REQUIRE(ph > 0, MFG_ERR_NUMERIC, "CG breakdown: operator not positive definite");
-      const double alpha = gh / ph;
-      res = std::sqrt(red.cg_update(pr, x, ws.r.ptr, ws.p.ptr, ws.h.ptr, alpha, n, s));
-      alphas.push_back(alpha);
+      op_vmult(A, v.h, v.p);
+      red.dot_dev(pr, v.p, v.h, n, kRedAlpha, 0, s);
+      red.cg_update_dev(pr, x, v.r, v.p, v.h, n, kRedSet, s);
+      red.read(sc, s);
+      MFG_REQUIRE(sc[kSlotBreakdown] == 0, MFG_ERR_NUMERIC, "CG breakdown: operator not positive definite");
+      if (it > 1) betas.push_back(sc[kSlotBeta]);
+      res = std::sqrt(sc[kSlotRR]);
+      alphas.push_back(sc[kSlotAlpha]);
       info->iterations = it;
-      if (res <= tol) break;
-      apply_precond(ws.z.ptr, ws.r.ptr);
-      const double ghn = red.dot(pr, ws.r.ptr, ws.z.ptr, n, s);
-      const double beta = ghn / gh;
-      gh = ghn;
-      betas.push_back(beta);
-      vec_xpby(pr, ws.p.ptr, ws.z.ptr, beta, n, s);
+      if (res <= tol) {
+        converged = true;
+        break;
+      }
+      apply_precond(v.z, v.r);
+      red.dot_dev(pr, v.r, v.z, n, kRedBeta, 0, s);
+      red.xpby_dev(pr, v.p, v.z, n, s);
+    }
+    if (!converged && info->iterations == max_iter) {
+      red.read(sc, s);
+      betas.push_back(sc[kSlotBeta]);
     }
   }
   info->final_residual = res;
@@ -641,7 +699,39 @@ void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, cons
     lanczos_estimate(alphas, betas, lmin, lmax);
     info->condition_estimate = lmin > 0 ? lmax / lmin : 0.0;
   }
-  if (res > tol) throw Error(MFG_ERR_NOCONV, "CG did not converge within the iteration limit");
+  if (!converged) throw Error(MFG_ERR_NOCONV, "CG did not converge within the iteration limit");
+}
+
+double cg_condition_estimate(const OperatorData &A, const void *b, void *x, int n_it, double tol, const CGVectors &v,
+                             Reducer &red) {
+  const int64_t n = A.n_dofs;
+  const int pr = A.precision;
+  hipStream_t s = A.stream;
+  if (n_it <= 0) return 1.0;
+  // x = 0, r = z = p = b; the stop flag ends the run at ||r|| <= tol
+  vec_fill(pr, x, n, 0.0, s);
+  vec_convert(pr, v.r, pr, b, n, s);
+  red.ensure_hist(n_it);
+  red.dot_dev(pr, v.r, v.r, n, kRedStart, 0, s);
+  vec_convert(pr, v.p, pr, v.r, n, s);
+  red.dot_dev(pr, v.r, v.r, n, kRedRes0, 0, s, tol, 0.0);
+  for (int it = 0; it < n_it; ++it) {
+    op_vmult(A, v.h, v.p);
+    red.dot_dev(pr, v.p, v.h, n, kRedAlpha, 0, s, 0, 0, true);
+    red.cg_update_dev(pr, x, v.r, v.p, v.h, n, kRedRRStop, s);
+    if (it + 1 == n_it) break;
+    red.dot_dev(pr, v.r, v.r, n, kRedBeta, 0, s, 0, 0, true);
+    red.xpby_dev(pr, v.p, v.r, n, s);
+  }
+  double sc[kRedScalars];
+  red.read(sc, s);
+  std::vector<double> alphas, betas;
+  red.read_hist(sc, alphas, betas, s);
+  if (alphas.empty()) return 1.0;
+  if (alphas.size() > betas.size() + 1) alphas.resize(betas.size() + 1);
+  double lmin, lmax;
+  lanczos_estimate(alphas, betas, lmin, lmax);
+  return lmin > 0 ? lmax / lmin : 0.0;
 }
 
 void CGWorkspace::ensure(size_t bytes) {

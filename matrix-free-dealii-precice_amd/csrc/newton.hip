@@ -52,10 +52,12 @@ static void residual_launch(const OperatorData &op, const double *u, const doubl
 }
 
 // R = residual(u, acc) (+ the interface traction over n_faces stored face
-// pairs) with constrained entries zero; returns ||R||
+// pairs) with constrained entries zero; returns ||R||.  One host round trip:
+// the norm, the det F flag and (sc) the reducer's scalar block come back
+// together.
 double solid_residual(const OperatorData &op, const double *u, const double *acc, const double *body, double *R,
                       Reducer &red, int64_t n_faces = 0, const int32_t *faces = nullptr,
-                      const double *traction = nullptr) {
+                      const double *traction = nullptr, double *sc = nullptr) {
   MFG_REQUIRE(op.kind == 0 && op.precision == MFG_FP64, MFG_ERR_ARGUMENT, "residual needs an FP64 solid operator");
   vec_fill(MFG_FP64, R, op.n_dofs, 0.0, op.stream);
   vec_fill(MFG_FP32, op.flag.ptr, 1, 0.0, op.stream);  // int 0 == float 0 bit pattern
@@ -70,12 +72,14 @@ double solid_residual(const OperatorData &op, const double *u, const double *acc
   }
   // mf_elasticity.h:1849-1897, skipping constrained dofs (zero after :1907)
   add_interface_traction(op, n_faces, faces, u, traction, R);
-  const double n = std::sqrt(red.dot(MFG_FP64, R, R, op.n_dofs, op.stream));
+  red.dot_dev(MFG_FP64, R, R, op.n_dofs, kRedSet, kSlotNewton, op.stream);
   int h = 0;
   MFG_HIP(hipMemcpyAsync(&h, op.flag.ptr, sizeof(int), hipMemcpyDeviceToHost, op.stream));
-  MFG_HIP(hipStreamSynchronize(op.stream));
+  double local[kRedScalars];
+  if (!sc) sc = local;
+  red.read(sc, op.stream);
   MFG_REQUIRE(h == 0, MFG_ERR_NUMERIC, "det_F is not positive");
-  return n;
+  return std::sqrt(sc[kSlotNewton]);
 }
 
 void SolidProblem::set_stream(hipStream_t s) {
@@ -112,13 +116,19 @@ void SolidProblem::step(mfg_step_info &info) {
   vec_fill(MFG_FP64, delta_u.ptr, N, 0.0, stream);
   double res0 = 1.0, upd = 1.0, upd0 = 1.0, res = 0.0;
   bool converged = false;
+  info.condition_estimate = 1.0;
+  double sc[kRedScalars];
   int it = 0;
   for (; it < nm.max_newton; ++it) {
     newmark_acc_kernel<double><<<grid, 256, 0, stream>>>(a.as<double>(), delta_u.as<double>(), v_old.as<double>(),
                                                          a_old.as<double>(), a1, a2, a3, N);
     MFG_HIP(hipGetLastError());
     res = solid_residual(F, u.as<double>(), a.as<double>(), nm.body_force, R.as<double>(), red, iface_n,
-                         iface_faces.as<int32_t>(), iface_traction);
+                         iface_faces.as<int32_t>(), iface_traction, sc);
+    if (it > 0) {  // ||du|| of the previous update came back with the residual
+      upd = std::sqrt(sc[kSlotUpd]);
+      if (it == 1) upd0 = upd;
+    }
     if (it == 0) {
       res0 = res;
       info.residual_0 = res;
@@ -132,14 +142,23 @@ void SolidProblem::step(mfg_step_info &info) {
       }
     }
     update_matrix_free();
+    ws.ensure((size_t)N * 8);
+    if (nm.cond_cg_iterations > 0)  // solve_linear_system's condition estimate (mf_elasticity.h:1998-2013)
+      info.condition_estimate =
+          cg_condition_estimate(F, R.ptr, du.ptr, nm.cond_cg_iterations, nm.tol_lin * res,
+                                CGVectors{ws.r.ptr, ws.z.ptr, ws.p.ptr, ws.h.ptr}, red);
     vec_fill(MFG_FP64, du.ptr, N, 0.0, stream);
     mfg_solve_info si{};
     cg_solve(F, MFG_PRECOND_MG, mg.get(), 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
     info.linear_iterations += si.iterations;
-    upd = std::sqrt(red.dot(MFG_FP64, du.ptr, du.ptr, N, stream));
-    if (it == 0) upd0 = upd;
+    red.dot_dev(MFG_FP64, du.ptr, du.ptr, N, kRedSet, kSlotUpd, stream);
     add2_kernel<double><<<grid, 256, 0, stream>>>(delta_u.as<double>(), u.as<double>(), du.as<double>(), N);
     MFG_HIP(hipGetLastError());
+  }
+  if (it == nm.max_newton && it > 0) {  // the last update's norm was not read with a residual
+    red.read(sc, stream);
+    upd = std::sqrt(sc[kSlotUpd]);
+    if (it == 1) upd0 = upd;
   }
   info.newton_iterations = it;
   info.residual_final = res;
@@ -205,6 +224,10 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
               "bad material");
   MFG_REQUIRE(material.formulation == 0 || material.formulation == 1, MFG_ERR_ARGUMENT,
               "Unknown material formulation");
+  MFG_REQUIRE(material.caching >= MFG_CACHING_SCALAR_REFERENTIAL && material.caching <= MFG_CACHING_TENSOR4_NS,
+              MFG_ERR_ARGUMENT, "Unknown caching strategy");
+  MFG_REQUIRE(material.formulation == 1 || material.caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
+              "formulation 0 needs the tensor2 caching");
   auto p = std::make_unique<SolidProblem>();
   p->L = n_levels;
   p->level_precision = level_precision;
@@ -219,7 +242,7 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
   p->a6 = (1 - gamma / (2 * beta)) * dt;
   mfg_solid_params sp = material;
   sp.alpha_1 = p->a1;
-  sp.caching = MFG_CACHING_TENSOR2;
+
   auto make_op = [&](const mfg_level_desc &d, int prec) {
     auto op = std::make_unique<mfg_operator_t>();
     op->kind = 0;
@@ -230,7 +253,8 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
     // the problem's operators never enter a batched launch, so its FP64
     // levels on affine cells take the 10-word point storage (faster and
     // smaller than the folded 16 words)
-    if (prec == MFG_FP64 && sp.formulation == 1 && op->affine_ref) op->t2_storage = MFG_T2_GRADIENT;
+    if (prec == MFG_FP64 && sp.formulation == 1 && sp.caching == MFG_CACHING_TENSOR2 && op->affine_ref)
+      op->t2_storage = MFG_T2_GRADIENT;
     op->setup_solid();
     return op;
   };

@@ -153,8 +153,15 @@ const char *mfg_version(void) { return "mfgpu 0.1 (gfx950)"; }
 
 int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n[3], int32_t degree,
                         int32_t n_components, int32_t face_mask, int32_t comp_mask, mfg_mesh *out) {
+  int32_t face_comp[6];
+  for (int f = 0; f < 6; ++f) face_comp[f] = ((face_mask >> f) & 1) ? comp_mask : 0;
+  return mfg_box_mesh_create_bc(lo, hi, n, degree, n_components, face_comp, out);
+}
+
+int mfg_box_mesh_create_bc(const double lo[3], const double hi[3], const int32_t n[3], int32_t degree,
+                           int32_t n_components, const int32_t face_comp_mask[6], mfg_mesh *out) {
   return guarded([&] {
-    MFG_REQUIRE(out && lo && hi && n, MFG_ERR_ARGUMENT, "null argument");
+    MFG_REQUIRE(out && lo && hi && n && face_comp_mask, MFG_ERR_ARGUMENT, "null argument");
     MFG_REQUIRE(degree >= 1 && degree <= kMaxDegree, MFG_ERR_ARGUMENT, "degree must be in 1..6");
     MFG_REQUIRE(n_components == 1 || n_components == 3, MFG_ERR_ARGUMENT, "n_components must be 1 or 3");
     auto m = new mfg_mesh_t();
@@ -209,16 +216,18 @@ int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n[
       m->node_x[id * 3 + 1] = line[1][J];
       m->node_x[id * 3 + 2] = line[2][K];
     }
-    // homogeneous Dirichlet dofs on the selected faces; sorted by construction
+    // homogeneous Dirichlet dofs: a node's component is constrained when any
+    // boundary face the node lies on constrains it (the union of the
+    // per-boundary-id masks, as make_constraints merges them); sorted by
+    // construction
     for (int64_t id = 0; id < m->n_nodes; ++id) {
       const int I = (int)(id % m->NN[0]), J = (int)((id / m->NN[0]) % m->NN[1]),
                 K = (int)(id / ((int64_t)m->NN[0] * m->NN[1]));
-      const bool on = ((face_mask & 1) && I == 0) || ((face_mask & 2) && I == m->NN[0] - 1) ||
-                      ((face_mask & 4) && J == 0) || ((face_mask & 8) && J == m->NN[1] - 1) ||
-                      ((face_mask & 16) && K == 0) || ((face_mask & 32) && K == m->NN[2] - 1);
-      if (!on) continue;
+      const int32_t comps = (I == 0 ? face_comp_mask[0] : 0) | (I == m->NN[0] - 1 ? face_comp_mask[1] : 0) |
+                            (J == 0 ? face_comp_mask[2] : 0) | (J == m->NN[1] - 1 ? face_comp_mask[3] : 0) |
+                            (K == 0 ? face_comp_mask[4] : 0) | (K == m->NN[2] - 1 ? face_comp_mask[5] : 0);
       for (int c = 0; c < n_components; ++c)
-        if (comp_mask & (1 << c)) m->constrained.push_back((int32_t)(id * n_components + c));
+        if (comps & (1 << c)) m->constrained.push_back((int32_t)(id * n_components + c));
     }
     *out = m;
   });

@@ -5,18 +5,39 @@
 
 namespace mfg {
 
-// Deterministic device reductions with a host read-back.
+// Deterministic device reductions (fixed grid, fixed-order sum).  The CG
+// forms keep their results and the step lengths derived from them in a block
+// of device scalars (RedSlot); the host reads that block back when it has a
+// decision to make.
 struct Reducer {
-  DeviceBuffer partials, result;
+  DeviceBuffer own;            // partials and scalars, when the reducer allocates them
+  DeviceBuffer hist_buf;       // step lengths of fixed-count runs (alphas, then betas)
+  double *partials = nullptr;  // kRedBlocks
+  double *scal = nullptr;      // kRedScalars
+  int hist_cap = 0;
   Reducer();
+  // Memory provided by the caller (kRedBlocks + kRedScalars doubles): the
+  // reducer allocates nothing.
+  Reducer(double *partials, double *scalars);
+  Reducer(const Reducer &) = delete;
+  Reducer &operator=(const Reducer &) = delete;
+  // a.b, read back at once
   double dot(int prec, const void *a, const void *b, int64_t n, hipStream_t s);
-  // x += alpha p; r -= alpha h; returns r.r
-  double cg_update(int prec, void *x, void *r, const void *p, const void *h, double alpha, int64_t n, hipStream_t s);
+  // a.b into the scalar block through the epilogue `mode` (RedMode)
+  void dot_dev(int prec, const void *a, const void *b, int64_t n, int mode, int slot, hipStream_t s,
+               double tol = 0, double rel = 0, bool record = false);
+  // x += alpha p; r -= alpha h (alpha from the scalar block); r.r through `mode`
+  void cg_update_dev(int prec, void *x, void *r, const void *p, const void *h, int64_t n, int mode, hipStream_t s);
+  // p = z + beta p (beta from the scalar block)
+  void xpby_dev(int prec, void *p, const void *z, int64_t n, hipStream_t s);
+  // the scalar block, synchronising the stream
+  void read(double *out, hipStream_t s);
+  void ensure_hist(int cap);
+  // recorded step lengths of the last fixed-count run (the counts from `sc`, a block read)
+  void read_hist(const double *sc, std::vector<double> &alphas, std::vector<double> &betas, hipStream_t s);
 
  private:
-  template <typename T> double dot_t(const void *a, const void *b, int64_t n, hipStream_t s);
-  template <typename T>
-  double cg_update_t(void *x, void *r, const void *p, const void *h, double alpha, int64_t n, hipStream_t s);
+  void epilogue(int mode, int slot, double tol, double rel, bool record, hipStream_t s);
 };
 
 void vec_fill(int prec, void *x, int64_t n, double v, hipStream_t s);
@@ -98,12 +119,31 @@ struct CGWorkspace {
   void ensure(size_t bytes);
 };
 
+// The four CG vectors, wherever they live.
+struct CGVectors {
+  void *r, *z, *p, *h;
+};
+
 // Extreme eigenvalues of the Lanczos matrix of CG coefficients.
 void lanczos_estimate(const std::vector<double> &alphas, const std::vector<double> &betas, double &lmin,
                       double &lmax);
 
 void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, const void *b, void *x, double tol_rel,
-              int max_iter, mfg_solve_info *info, CGWorkspace &ws, Reducer &red);
+              int max_iter, mfg_solve_info *info, const CGVectors &v, Reducer &red);
+inline void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, const void *b, void *x,
+                     double tol_rel, int max_iter, mfg_solve_info *info, CGWorkspace &ws, Reducer &red) {
+  ws.ensure((size_t)A.n_dofs * A.elem_size());
+  cg_solve(A, precond, mg, omega, b, x, tol_rel, max_iter, info, CGVectors{ws.r.ptr, ws.z.ptr, ws.p.ptr, ws.h.ptr},
+           red);
+}
+
+// Unpreconditioned CG from x = 0 for at most n_it iterations or until
+// ||r|| <= tol (deal.II's IterationNumberControl), run without host round
+// trips; returns the condition estimate lmax / lmin of its Lanczos matrix
+// (1 when no step was taken).  x is left at the iterate; the vectors are
+// scratch.
+double cg_condition_estimate(const OperatorData &A, const void *b, void *x, int n_it, double tol, const CGVectors &v,
+                             Reducer &red);
 
 }  // namespace mfg
 

@@ -90,6 +90,149 @@ static __global__ void __launch_bounds__(kRedThreads) sum_partials_kernel(const 
   if (threadIdx.x == 0) *out = s[0];
 }
 
+// ------------------------------------------ device-resident CG scalars ---
+// A block of scalars that the reduction epilogues and the update kernels read
+// and write, so that a CG iteration computes its step lengths on the device
+// and the host copies back only what it decides on (the residual norm, once
+// per iteration) or nothing at all (fixed-count runs, stopped by a flag).
+enum RedSlot : int {
+  kSlotGH = 0,     // (r, z) of the current iterate
+  kSlotPH,         // (p, A p)
+  kSlotAlpha,      // step length of this iteration; 0 when the iteration is inactive
+  kSlotBeta,       // direction-update coefficient of the last active iteration
+  kSlotRR,         // (r, r) after the update
+  kSlotB2,         // (b, b)
+  kSlotRes0,       // ||r_0|| of a fixed-count run
+  kSlotTol,        // a fixed-count run stops at ||r|| <= max(tol, rel ||r_0||)
+  kSlotRel,
+  kSlotStop,       // 1: later iterations change neither x, r nor the history
+  kSlotActive,     // 1: this iteration took a step
+  kSlotBreakdown,  // 1: (p, A p) <= 0 was met
+  kSlotNAlpha,     // history lengths (integers stored as doubles)
+  kSlotNBeta,
+  kSlotHost,       // host-synchronous dot products
+  kSlotUpd,        // (du, du) of the Newton update, read with the next residual
+  kSlotNewton,     // (R, R) of the Newton residual
+  kRedScalars = 32
+};
+// Epilogue of the final reduction pass (v = the reduced sum):
+//   kRedSet     scal[slot] = v
+//   kRedStart   (r, z) of the first iterate: resets the flags and the history
+//   kRedRes0    ||r_0|| = sqrt(v) and the stop threshold (tol, rel); stop at once if ||r_0|| <= tol
+//   kRedAlpha   (p, A p): alpha = gh / v and record it, or stop on breakdown
+//   kRedRRStop  (r, r) after the update; stop when below the threshold
+//   kRedBeta    (r, z) of the new iterate: beta = v / gh and record it
+enum RedMode : int { kRedSet = 0, kRedStart, kRedRes0, kRedAlpha, kRedRRStop, kRedBeta };
+
+__device__ inline void red_epilogue(double v, double *scal, int mode, int slot, double tol, double rel,
+                                    double *hist, int hist_cap) {
+  switch (mode) {
+    case kRedSet: scal[slot] = v; break;
+    case kRedStart:
+      scal[kSlotGH] = v;
+      scal[kSlotStop] = 0;
+      scal[kSlotActive] = 1;
+      scal[kSlotBreakdown] = 0;
+      scal[kSlotNAlpha] = 0;
+      scal[kSlotNBeta] = 0;
+      scal[kSlotTol] = 0;
+      scal[kSlotRel] = 0;
+      scal[kSlotAlpha] = 0;
+      scal[kSlotBeta] = 0;
+      break;
+    case kRedRes0: {
+      const double r0 = sqrt(v);
+      scal[kSlotRes0] = r0;
+      scal[kSlotTol] = tol;
+      scal[kSlotRel] = rel;
+      if (!(r0 > tol)) scal[kSlotStop] = 1;
+      break;
+    }
+    case kRedAlpha:
+      scal[kSlotPH] = v;
+      if (scal[kSlotStop] != 0) {
+        scal[kSlotAlpha] = 0;
+        scal[kSlotActive] = 0;
+      } else if (!(v > 0)) {
+        scal[kSlotAlpha] = 0;
+        scal[kSlotActive] = 0;
+        scal[kSlotStop] = 1;
+        scal[kSlotBreakdown] = 1;
+      } else {
+        const double a = scal[kSlotGH] / v;
+        scal[kSlotAlpha] = a;
+        scal[kSlotActive] = 1;
+        const int k = (int)scal[kSlotNAlpha];
+        if (hist && k < hist_cap) {
+          hist[k] = a;
+          scal[kSlotNAlpha] = k + 1;
+        }
+      }
+      break;
+    case kRedRRStop:
+      scal[kSlotRR] = v;
+      if (scal[kSlotActive] != 0 && sqrt(v) <= fmax(scal[kSlotTol], scal[kSlotRel] * scal[kSlotRes0]))
+        scal[kSlotStop] = 1;
+      break;
+    case kRedBeta:
+      if (scal[kSlotActive] != 0 && scal[kSlotStop] == 0) {
+        const double b = v / scal[kSlotGH];
+        scal[kSlotGH] = v;
+        scal[kSlotBeta] = b;
+        const int k = (int)scal[kSlotNBeta];
+        if (hist && k < hist_cap) {
+          hist[hist_cap + k] = b;
+          scal[kSlotNBeta] = k + 1;
+        }
+      } else {
+        scal[kSlotBeta] = 0;
+      }
+      break;
+  }
+}
+
+static __global__ void __launch_bounds__(kRedThreads)
+    sum_partials_epilogue_kernel(const double *partials, int n, double *scal, int mode, int slot, double tol,
+                                 double rel, double *hist, int hist_cap) {
+  __shared__ double s[kRedThreads];
+  double v = 0;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) v += partials[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red_epilogue(s[0], scal, mode, slot, tol, rel, hist, hist_cap);
+}
+
+// x += alpha p ; r -= alpha h with alpha = scal[kSlotAlpha] (no change when it
+// is 0, i.e. the iteration is inactive) ; partial r.r
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) cg_update_dev_kernel(T *x, T *r, const T *p, const T *h,
+                                                                     const double *scal, int64_t n,
+                                                                     double *partials) {
+  const double alpha = scal[kSlotAlpha];
+  double v = 0;
+  MFG_GRID_STRIDE(i, n) {
+    T ri = r[i];
+    if (alpha != 0) {
+      x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+      ri = (T)((double)ri - alpha * (double)h[i]);
+      r[i] = ri;
+    }
+    v += (double)ri * (double)ri;
+  }
+  block_reduce_store<T>(v, partials);
+}
+
+// p = d + beta p with beta = scal[kSlotBeta]
+template <typename T>
+__global__ void xpby_dev_kernel(T *p, const T *d, const double *scal, int64_t n) {
+  const double beta = scal[kSlotBeta];
+  MFG_GRID_STRIDE(i, n) p[i] = (T)((double)d[i] + beta * (double)p[i]);
+}
+
 // x += alpha p ; r -= alpha h ; partial r.r
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads) cg_update_kernel(T *x, T *r, const T *p, const T *h, double alpha,

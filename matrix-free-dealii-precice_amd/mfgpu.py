@@ -50,6 +50,8 @@ SIGNATURES = {
     "mfg_operator_cell_schedule": (ctypes.c_int32, [ctypes.c_void_p]),
     "mfg_last_error": (ctypes.c_char_p, []),
     "mfg_version": (ctypes.c_char_p, []),
+    "mfg_box_mesh_create_bc": (ctypes.c_int, [_c_dp, _c_dp, _c_i32p, ctypes.c_int32, ctypes.c_int32, _c_i32p,
+                                               ctypes.POINTER(_vp)]),
     "mfg_box_mesh_create": (ctypes.c_int, [_c_dp, _c_dp, _c_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, ctypes.POINTER(_vp)]),
     "mfg_box_mesh_destroy": (ctypes.c_int, [_vp]),
@@ -180,15 +182,21 @@ def _check_pair(dst, src, n, dtype):
 class BoxMesh:
     """Structured box mesh level (cells, nodes, Dirichlet dofs) built on the host."""
 
-    def __init__(self, lo, hi, n_cells, degree, n_components, face_mask=1, comp_mask=7, numbering="lexicographic"):
+    def __init__(self, lo, hi, n_cells, degree, n_components, face_mask=1, comp_mask=7, numbering="lexicographic",
+                 face_comp=None):
+        """face_comp: optional component mask per boundary face (6 ints: x=lo, x=hi, y=lo, y=hi, z=lo,
+        z=hi), which replaces (face_mask, comp_mask)."""
         import numpy as np
         self._lo = np.ascontiguousarray(lo, dtype=np.float64)
         self._hi = np.ascontiguousarray(hi, dtype=np.float64)
         self._n = np.ascontiguousarray(n_cells, dtype=np.int32)
+        if face_comp is None:
+            face_comp = [comp_mask if (face_mask >> f) & 1 else 0 for f in range(6)]
+        self._face_comp = np.ascontiguousarray(face_comp, dtype=np.int32)
         h = _vp()
-        check(lib().mfg_box_mesh_create(self._lo.ctypes.data_as(_c_dp), self._hi.ctypes.data_as(_c_dp),
-                                         self._n.ctypes.data_as(_c_i32p), degree, n_components, face_mask,
-                                         comp_mask, ctypes.byref(h)))
+        check(lib().mfg_box_mesh_create_bc(self._lo.ctypes.data_as(_c_dp), self._hi.ctypes.data_as(_c_dp),
+                                            self._n.ctypes.data_as(_c_i32p), degree, n_components,
+                                            self._face_comp.ctypes.data_as(_c_i32p), ctypes.byref(h)))
         self.h = h
         self._destroy = lib().mfg_box_mesh_destroy
         if numbering not in NUMBERING:
@@ -553,13 +561,15 @@ class NewmarkParams(ctypes.Structure):
     _fields_ = [("beta", ctypes.c_double), ("gamma", ctypes.c_double), ("dt", ctypes.c_double),
                 ("body_force", ctypes.c_double * 3), ("max_newton", ctypes.c_int32), ("min_newton", ctypes.c_int32),
                 ("tol_f", ctypes.c_double), ("tol_u", ctypes.c_double), ("tol_f_abs", ctypes.c_double),
-                ("tol_u_abs", ctypes.c_double), ("tol_lin", ctypes.c_double), ("max_lin", ctypes.c_int32)]
+                ("tol_u_abs", ctypes.c_double), ("tol_lin", ctypes.c_double), ("max_lin", ctypes.c_int32),
+                ("cond_cg_iterations", ctypes.c_int32)]
 
 
 class StepInfo(ctypes.Structure):
     _fields_ = [("newton_iterations", ctypes.c_int32), ("linear_iterations", ctypes.c_int32),
                 ("converged", ctypes.c_int32), ("residual_0", ctypes.c_double), ("residual_final", ctypes.c_double),
-                ("update_final", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("update_final", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("condition_estimate", ctypes.c_double)]
 
 
 SIGNATURES.update({
@@ -619,7 +629,7 @@ SIGNATURES.update({
 def session_spec(lo, hi, n_cells, degree, mu, nu, rho, beta=0.25, gamma=0.5, dt=1e-3, body=(0.0, 0.0, 0.0),
                  level_precision=FP32, tol_f=1e-8, tol_u=1e-6, tol_lin=1e-6, max_newton=10, min_newton=3,
                  cheb_degree=5, smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0, u0_amplitude=1e-3,
-                 u0_noise=1e-5, seed=0):
+                 u0_noise=1e-5, seed=0, cond_cg_iterations=20):
     s = SessionSpec()
     s.lo[:] = list(lo)
     s.hi[:] = list(hi)
@@ -627,7 +637,7 @@ def session_spec(lo, hi, n_cells, degree, mu, nu, rho, beta=0.25, gamma=0.5, dt=
     s.degree = degree
     s.material = SolidParams(mu, nu, rho, 0.0, 1, CACHING["tensor2"])
     s.newmark = NewmarkParams(beta, gamma, dt, (ctypes.c_double * 3)(*body), max_newton, min_newton, tol_f, tol_u,
-                              0.0, 0.0, tol_lin, 1000)
+                              0.0, 0.0, tol_lin, 1000, cond_cg_iterations)
     s.mg = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, 1e-3, 1e-3)
     s.level_precision = level_precision
     s.u0_amplitude, s.u0_noise, s.seed = u0_amplitude, u0_noise, seed
@@ -666,15 +676,15 @@ class SolidProblem:
 
     def __init__(self, meshes, mu, nu, rho, beta=0.25, gamma=0.5, dt=1e-3, body=(0.0, 0.0, 0.0), formulation=1,
                  level_precision=FP64, max_newton=10, min_newton=3, tol_f=1e-8, tol_u=1e-6, tol_f_abs=0.0,
-                 tol_u_abs=0.0, tol_lin=1e-6, max_lin=1000, cheb_degree=5, smoothing_range=20.0,
-                 eig_cg_iterations=30, coarse_kind=0, coarse_range=1e-3, coarse_tolerance=1e-3):
+                 tol_u_abs=0.0, tol_lin=1e-6, max_lin=1000, cond_cg_iterations=20, cheb_degree=5, smoothing_range=20.0,
+                 eig_cg_iterations=30, coarse_kind=0, coarse_range=1e-3, coarse_tolerance=1e-3, caching="tensor2"):
         L = len(meshes)
         descs = (LevelDesc * L)(*[m.desc for m in meshes])
         self._children = [box_children(meshes[l - 1], meshes[l]) for l in range(1, L)]
         ch = (_c_i32p * max(L - 1, 1))(*([c.ctypes.data_as(_c_i32p) for c in self._children] or [None]))
-        sp = SolidParams(mu, nu, rho, 0.0, formulation, CACHING["tensor2"])
+        sp = SolidParams(mu, nu, rho, 0.0, formulation, CACHING[caching])
         nm = NewmarkParams(beta, gamma, dt, (ctypes.c_double * 3)(*body), max_newton, min_newton, tol_f, tol_u,
-                           tol_f_abs, tol_u_abs, tol_lin, max_lin)
+                           tol_f_abs, tol_u_abs, tol_lin, max_lin, cond_cg_iterations)
         mp = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, coarse_range, coarse_tolerance)
         h = _vp()
         check(lib().mfg_solid_problem_create(L, descs, ch, ctypes.byref(sp), ctypes.byref(nm), ctypes.byref(mp),

@@ -119,7 +119,6 @@ struct Mesh {
   vector<int32_t> constrained;   // sorted dofs
   vector<char> is_con;           // per dof
   vector<double> xnode;          // n_nodes * 3
-  int face_mask = 0, comp_mask = 0;
 
   int64_t node_index(int I, int J, int K) const {
     return (int64_t)I + (int64_t)NN[0] * ((int64_t)J + (int64_t)NN[1] * K);
@@ -132,10 +131,12 @@ struct Mesh {
   }
 };
 
+// face_comp[f]: components constrained on boundary face f (0:x=lo, 1:x=hi,
+// 2:y=lo, 3:y=hi, 4:z=lo, 5:z=hi); a node on several faces takes the union
 inline void build_mesh(Mesh &m, const double *lo, const double *hi, const int *n, int p, int nc,
-                       int face_mask, int comp_mask) {
+                       const int *face_comp) {
   if (p < 1 || p > 8) throw std::runtime_error("degree out of range");
-  m.p = p; m.N = p + 1; m.nc = nc; m.face_mask = face_mask; m.comp_mask = comp_mask;
+  m.p = p; m.N = p + 1; m.nc = nc;
   for (int d = 0; d < 3; ++d) {
     m.n[d] = n[d]; m.lo[d] = lo[d]; m.hi[d] = hi[d];
     m.h[d] = (hi[d] - lo[d]) / n[d];
@@ -178,17 +179,17 @@ inline void build_mesh(Mesh &m, const double *lo, const double *hi, const int *n
   for (int K = 0; K < m.NN[2]; ++K)
     for (int J = 0; J < m.NN[1]; ++J)
       for (int I = 0; I < m.NN[0]; ++I) {
-        bool on = false;
-        if ((face_mask & 1) && I == 0) on = true;
-        if ((face_mask & 2) && I == m.NN[0] - 1) on = true;
-        if ((face_mask & 4) && J == 0) on = true;
-        if ((face_mask & 8) && J == m.NN[1] - 1) on = true;
-        if ((face_mask & 16) && K == 0) on = true;
-        if ((face_mask & 32) && K == m.NN[2] - 1) on = true;
-        if (!on) continue;
+        int comps = 0;
+        if (I == 0) comps |= face_comp[0];
+        if (I == m.NN[0] - 1) comps |= face_comp[1];
+        if (J == 0) comps |= face_comp[2];
+        if (J == m.NN[1] - 1) comps |= face_comp[3];
+        if (K == 0) comps |= face_comp[4];
+        if (K == m.NN[2] - 1) comps |= face_comp[5];
+        if (!comps) continue;
         int64_t id = m.node_index(I, J, K);
         for (int c = 0; c < nc; ++c)
-          if (comp_mask & (1 << c)) m.is_con[id * nc + c] = 1;
+          if (comps & (1 << c)) m.is_con[id * nc + c] = 1;
       }
   m.constrained.clear();
   for (int64_t i = 0; i < m.n_dofs; ++i)

@@ -480,6 +480,13 @@ int orc_mg_cg_solve(int kind, int n_levels, void **meshes, void **ops, const orc
 // nm = {beta, gamma, dt, bx, by, bz, tol_f, tol_u, tol_f_abs, tol_u_abs, tol_lin}
 int orc_newmark_step(int n_levels, void **meshes, void **ops, const orc_mg_params *prm, const double *nm,
                      int max_newton, int min_newton, double *u, double *v_old, double *a_old, int *lin_its) {
+  return orc_newmark_step_iface(n_levels, meshes, ops, prm, nm, max_newton, min_newton, u, v_old, a_old, lin_its, 0,
+                                nullptr, nullptr);
+}
+
+int orc_newmark_step_iface(int n_levels, void **meshes, void **ops, const orc_mg_params *prm, const double *nm,
+                           int max_newton, int min_newton, double *u, double *v_old, double *a_old, int *lin_its,
+                           int64_t n_faces, const int32_t *faces, const double *traction) {
   const double beta = nm[0], gamma = nm[1], dt = nm[2];
   const double a1 = 1.0 / (beta * dt * dt), a2 = 1.0 / (beta * dt), a3 = (1 - 2 * beta) / (2 * beta);
   const double a4 = gamma / (beta * dt), a5 = 1 - gamma / beta, a6 = (1 - gamma / (2 * beta)) * dt;
@@ -495,6 +502,8 @@ int orc_newmark_step(int n_levels, void **meshes, void **ops, const orc_mg_param
   for (; it < max_newton; ++it) {
     for (size_t i = 0; i < n; ++i) acc[i] = a1 * delta[i] - a2 * v_old[i] - a3 * a_old[i];
     orc_solid_residual(ops[n_levels - 1], u, acc.data(), body, R.data());
+    // mf_elasticity.h:1849-1897: the interface traction, pulled back at u
+    if (n_faces) orc_interface_traction(&fm, n_faces, faces, u, traction, R.data());
     const double res = std::sqrt(dot(R, R));
     if (it == 0) res0 = res;
     if (it >= min_newton) {

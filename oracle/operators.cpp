@@ -680,8 +680,13 @@ void orc_lagrange(int m, const double *z, int nx, const double *x, double *val, 
 
 void *orc_mesh_create(const double *lo, const double *hi, const int *n, int p, int nc, int face_mask,
                       int comp_mask) {
+  int face_comp[6];
+  for (int f = 0; f < 6; ++f) face_comp[f] = ((face_mask >> f) & 1) ? comp_mask : 0;
+  return orc_mesh_create_bc(lo, hi, n, p, nc, face_comp);
+}
+void *orc_mesh_create_bc(const double *lo, const double *hi, const int *n, int p, int nc, const int *face_comp) {
   Mesh *m = new Mesh();
-  build_mesh(*m, lo, hi, n, p, nc, face_mask, comp_mask);
+  build_mesh(*m, lo, hi, n, p, nc, face_comp);
   return m;
 }
 void orc_mesh_destroy(void *m) { delete (Mesh *)m; }

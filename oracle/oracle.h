@@ -46,6 +46,12 @@ void orc_lagrange(int m, const double *z, int nx, const double *x, double *val,
  *  3*(ny*p+1)*(nz*p+1); tests/test_abi.py, tests/test_golden.py] */
 void *orc_mesh_create(const double *lo, const double *hi, const int *n, int p,
                       int nc, int face_mask, int comp_mask);
+/* The same with a component mask per face (face_comp[6]); a node on several
+ * faces is constrained in the union of their masks.
+ * [pinned: per-face constrained counts in closed form, bit-identical to the
+ *  device builder; tests/test_turek.py] */
+void *orc_mesh_create_bc(const double *lo, const double *hi, const int *n, int p,
+                         int nc, const int *face_comp);
 void orc_mesh_destroy(void *m);
 /* out[0..5] = n_cells, n_nodes, n_dofs, n_constrained, nodes_per_cell, nc */
 void orc_mesh_sizes(void *m, int64_t *out);
@@ -183,6 +189,16 @@ int orc_newmark_step(int n_levels, void **meshes, void **ops,
                      const orc_mg_params *prm, const double *nm,
                      int max_newton, int min_newton, double *u, double *v_old,
                      double *a_old, int *lin_its);
+/* The same with an interface: every residual adds the traction term over
+ * n_faces (cell, face) pairs of the finest mesh (orc_interface_traction; t:
+ * 3 values per face point).
+ * [pinned: the reference's own goldens -- Newton counts and watchpoint
+ *  displacements of the Turek flap, tests/test_turek.py] */
+int orc_newmark_step_iface(int n_levels, void **meshes, void **ops,
+                           const orc_mg_params *prm, const double *nm,
+                           int max_newton, int min_newton, double *u, double *v_old,
+                           double *a_old, int *lin_its, int64_t n_faces,
+                           const int32_t *faces, const double *traction);
 /* plain / Jacobi-preconditioned CG (omega scales the inverse diagonal) */
 int orc_cg_solve(int kind, void *op, int precond, double omega, const double *b,
                  double *x, double tol_rel, int max_iter, double *final_rel);

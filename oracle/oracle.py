@@ -36,6 +36,10 @@ def lib():
             "orc_gauss_rule": (None, [ctypes.c_int, _dp, _dp]),
             "orc_lagrange": (None, [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]),
             "orc_mesh_create": (_vp, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "orc_mesh_create_bc": (_vp, [_dp, _dp, _ip, ctypes.c_int, ctypes.c_int, _ip]),
+            "orc_newmark_step_iface": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                      ctypes.POINTER(MGParams), _dp, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                                      _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int64, _ip, _dp]),
             "orc_mesh_destroy": (None, [_vp]),
             "orc_mesh_sizes": (None, [_vp, _lp]),
             "orc_mesh_cell_nodes": (None, [_vp, _ip]),
@@ -133,13 +137,17 @@ def fill_uniform(seed, n):
 
 
 class Mesh:
-    def __init__(self, lo, hi, n, p, nc, face_mask=1, comp_mask=7):
+    def __init__(self, lo, hi, n, p, nc, face_mask=1, comp_mask=7, face_comp=None):
+        """face_comp: optional component mask per boundary face (6 ints), which
+        replaces (face_mask, comp_mask)."""
         lo = np.ascontiguousarray(lo, dtype=np.float64)
         hi = np.ascontiguousarray(hi, dtype=np.float64)
         nn = np.ascontiguousarray(n, dtype=np.int32)
         self.lo, self.hi, self.n, self.p, self.nc = lo, hi, list(n), p, nc
-        self.face_mask, self.comp_mask = face_mask, comp_mask
-        self.h = lib().orc_mesh_create(dptr(lo), dptr(hi), iptr(nn), p, nc, face_mask, comp_mask)
+        if face_comp is None:
+            face_comp = [comp_mask if (face_mask >> f) & 1 else 0 for f in range(6)]
+        self.face_comp = np.ascontiguousarray(face_comp, dtype=np.int32)
+        self.h = lib().orc_mesh_create_bc(dptr(lo), dptr(hi), iptr(nn), p, nc, iptr(self.face_comp))
         s = np.zeros(6, dtype=np.int64)
         lib().orc_mesh_sizes(self.h, s.ctypes.data_as(_lp))
         self.n_cells, self.n_nodes, self.n_dofs, self.n_constrained, self.nodes_per_cell = (int(v) for v in s[:5])
@@ -359,16 +367,26 @@ def mg_vmult(meshes, ops, src, cheb_degree=5, smoothing_range=20.0, eig_cg_itera
 
 def newmark_step(meshes, ops, u, v, a, beta=BETA, gamma=0.5, dt=DT, body=(0.0, 0.0, 0.0), tol_f=1e-8, tol_u=1e-6,
                  tol_f_abs=0.0, tol_u_abs=0.0, tol_lin=1e-6, max_newton=10, min_newton=3, cheb_degree=5,
-                 smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0):
-    """One Newton-Newmark step in place on (u, v, a); returns (newton_its, linear_its)."""
+                 smoothing_range=20.0, eig_cg_iterations=30, coarse_kind=0, faces=None, traction=None):
+    """One Newton-Newmark step in place on (u, v, a); returns (newton_its, linear_its).  faces / traction:
+    an interface ((cell, face) pairs of the finest mesh, 3 traction values per face point) whose traction
+    term every residual adds."""
     L = len(meshes)
     mh = (_vp * L)(*[m.h for m in meshes])
     oh = (_vp * L)(*[o.h for o in ops])
     prm = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, 1e-6, 1000)
     nm = np.array([beta, gamma, dt, *body, tol_f, tol_u, tol_f_abs, tol_u_abs, tol_lin], dtype=np.float64)
     li = ctypes.c_int(0)
-    it = lib().orc_newmark_step(L, mh, oh, ctypes.byref(prm), dptr(nm), max_newton, min_newton, dptr(u), dptr(v),
-                                dptr(a), ctypes.byref(li))
+    if faces is None:
+        f = np.zeros(2, dtype=np.int32)
+        t = np.zeros(3)
+        nf = 0
+    else:
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        t = np.ascontiguousarray(traction, dtype=np.float64).reshape(-1)
+        nf = len(f) // 2
+    it = lib().orc_newmark_step_iface(L, mh, oh, ctypes.byref(prm), dptr(nm), max_newton, min_newton, dptr(u),
+                                      dptr(v), dptr(a), ctypes.byref(li), nf, iptr(f), dptr(t))
     return it, li.value
 
 
@@ -400,6 +418,7 @@ def cheb_dense(A, b, degree, smoothing_range, eig_it, x0=None):
     return x, dict(lmax=info[0], lmin=info[1], theta=info[2], delta=info[3], degree=int(info[4]))
 
 
-def box_levels(lo, hi, coarse_n, n_levels, p, nc, face_mask=1, comp_mask=7):
+def box_levels(lo, hi, coarse_n, n_levels, p, nc, face_mask=1, comp_mask=7, face_comp=None):
     """Nested structured boxes: level l has coarse_n * 2^l cells per direction."""
-    return [Mesh(lo, hi, [c << l for c in coarse_n], p, nc, face_mask, comp_mask) for l in range(n_levels)]
+    return [Mesh(lo, hi, [c << l for c in coarse_n], p, nc, face_mask, comp_mask, face_comp=face_comp)
+            for l in range(n_levels)]

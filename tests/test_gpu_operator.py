@@ -391,8 +391,9 @@ def test_error_paths(cuda):
     gm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
     go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
     y = torch.zeros(gm.n_dofs, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(y)
     with pytest.raises(mfgpu.MfgError) as e:
-        go.vmult(y, y)  # before cache()
+        go.vmult(y, x)  # before cache()
     assert e.value.code == 5
     # det F <= 0: a displacement that inverts cells
     om = O.Mesh([0, 0, 0], [1, 1, 1], [2, 2, 2], 2, 3, 1, 7)
@@ -472,8 +473,9 @@ def test_gradient_point_storage_errors(cuda):
     mem16 = go.memory_consumption()
     go.set_point_storage(mfgpu.T2_GRADIENT)
     y = torch.zeros(gm.n_dofs, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(y)
     with pytest.raises(mfgpu.MfgError) as e:  # the change discards the cache
-        go.vmult(y, y)
+        go.vmult(y, x)
     assert e.value.code == 5
     go.cache(u)
     assert go.memory_consumption() < mem16
