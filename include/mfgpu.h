@@ -360,6 +360,19 @@ typedef struct mfg_solve_info {
 } mfg_solve_info;
 int mfg_cg_solve(mfg_operator A, int32_t preconditioner, mfg_mg mg, double omega, const void *b, void *x,
                  double tol_rel, int32_t max_iter, mfg_solve_info *info);
+/* mfg_cg_solve without device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The CG vectors, reduction
+ * partials and step-length scalars live in a file-scope device array of
+ * 32 MiB + 8 KiB that each GPU the library runs on holds until the process
+ * ends; the first call on a device looks up its address (and loads the module
+ * holding it).  Limit: operators of at most 2^20 rows (MFG_ERR_ARGUMENT
+ * beyond); the multigrid preconditioner must not replay graphs
+ * (mfg_mg_use_graph off).  Every call on one GPU shares that array: calls
+ * that overlap on one GPU race.  Same arguments, results and errors as
+ * mfg_cg_solve. */
+int mfg_cg_solve_noalloc(mfg_operator A, int32_t preconditioner, mfg_mg mg, double omega, const void *b, void *x,
+                         double tol_rel, int32_t max_iter, mfg_solve_info *info);
 
 /* ------------------------------------------------ Newton-Newmark driver --- */
 /* Residual of the solid at displacement u and acceleration acc, on the

@@ -14,7 +14,7 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libmfgpu.so")
 
 SOURCES = ["setup.cpp", "operator.hip", "vectors.hip", "multigrid.hip", "newton.hip", "batched.hip", "serving.cpp",
-           "heat_problem.hip", "interface.hip"]
+           "heat_problem.hip", "interface.hip", "cg_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from forming v_pk_* FP32 instructions.

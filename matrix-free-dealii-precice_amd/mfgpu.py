@@ -109,6 +109,8 @@ SIGNATURES = {
     "mfg_mg_level_info": (ctypes.c_int, [_vp, ctypes.c_int32, _c_dp, _c_dp, _c_i32p]),
     "mfg_cg_solve": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.c_double, _vp, _vp, ctypes.c_double,
                                      ctypes.c_int32, ctypes.c_void_p]),
+    "mfg_cg_solve_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.c_double, _vp, _vp, ctypes.c_double,
+                                             ctypes.c_int32, ctypes.c_void_p]),
     "mfg_fill_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
     "mfg_box_mesh_fill_u0": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _c_dp]),
 }
