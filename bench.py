@@ -292,7 +292,11 @@ def serving_bench(mfgpu, n_sessions=32, n_requests=64, rate=40.0, seed=100, max_
 
     res, st, _ = mfgpu.serve(specs(mfgpu.FP32), reqs, max_concurrent=max_concurrent, real_time=True)
     log(f"serving: {st.completed} steps, {st.steps_per_s:.1f} steps/s, p50 {st.latency_p50_s * 1e3:.1f} ms")
-    res64, st64, _ = mfgpu.serve(specs(mfgpu.FP64), reqs, max_concurrent=1, real_time=False)
+    # capacity: the same requests all released at t = 0 (saturation)
+    sat = [(s, 0.0) for s, _ in reqs]
+    res_c, st_c, _ = mfgpu.serve(specs(mfgpu.FP32), sat, max_concurrent=max_concurrent, real_time=False)
+    log(f"serving: capacity {st_c.steps_per_s:.1f} steps/s")
+    res64, st64, _ = mfgpu.serve(specs(mfgpu.FP64), sat, max_concurrent=1, real_time=False)
     log(f"serving: sequential FP64 {st64.steps_per_s:.1f} steps/s")
     return {
         "workload": f"{n_sessions} solid sessions (p ~ U{{2..4}}, beam n_x ~ U{{8..48}} step 4, n_y = n_z = n_x/4), "
@@ -304,9 +308,15 @@ def serving_bench(mfgpu, n_sessions=32, n_requests=64, rate=40.0, seed=100, max_
                    "evictions": st.evictions, "max_resident": st.max_resident,
                    "peak_bytes": int(st.peak_bytes), "max_concurrent": max_concurrent,
                    "level_precision": "fp32 (fp64 coarse, outer CG and Newton)"},
-        "sequential_fp64": {"steps_per_s": round(st64.steps_per_s, 2), "makespan_s": round(st64.makespan_s, 3),
+        "capacity": {"workload": "the same requests, all released at t = 0", "steps_per_s": round(st_c.steps_per_s, 2),
+                     "latency_p50_ms": round(st_c.latency_p50_s * 1e3, 2),
+                     "latency_p99_ms": round(st_c.latency_p99_s * 1e3, 2), "makespan_s": round(st_c.makespan_s, 3),
+                     "completed": st_c.completed, "failed": st_c.failed, "max_concurrent": max_concurrent},
+        "sequential_fp64": {"workload": "the same requests, all released at t = 0, one at a time, FP64 levels",
+                            "steps_per_s": round(st64.steps_per_s, 2), "makespan_s": round(st64.makespan_s, 3),
                             "latency_p50_ms": round(st64.latency_p50_s * 1e3, 2), "failed": st64.failed},
         "throughput_vs_sequential_fp64": round(st.steps_per_s / max(st64.steps_per_s, 1e-30), 3),
+        "capacity_vs_sequential_fp64": round(st_c.steps_per_s / max(st64.steps_per_s, 1e-30), 3),
         "newton_iterations_mean": round(float(np.mean([r.newton_iterations for r in res])), 2),
         "linear_iterations_mean": round(float(np.mean([r.linear_iterations for r in res])), 2),
     }

@@ -483,7 +483,7 @@ typedef struct mfg_request_result {
 typedef struct mfg_server_params {
   int32_t max_concurrent;
   size_t hbm_budget_bytes;
-  int32_t real_time;  /* 1: honour arrival times; 0: all requests available at t = 0 */
+  int32_t real_time;  /* 1: honour arrival times; 0: all requests available at t = 0 (latency from t = 0) */
 } mfg_server_params;
 typedef struct mfg_serve_stats {
   double makespan_s, steps_per_s, latency_p50_s, latency_p99_s, latency_mean_s;

@@ -238,7 +238,9 @@ struct Server {
     for (int i = 0; i < n; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return requests[a].arrival_s < requests[b].arrival_s; });
-    for (int i = 0; i < n; ++i) results[i].arrival_s = requests[i].arrival_s;
+    // latency runs from the release: the arrival time in real time, t = 0
+    // when every request is released at once
+    for (int i = 0; i < n; ++i) results[i].arrival_s = prm.real_time ? requests[i].arrival_s : 0.0;
     t0 = Clock::now();
     std::vector<std::thread> workers;
     const int W = std::max(1, prm.max_concurrent);

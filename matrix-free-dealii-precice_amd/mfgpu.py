@@ -877,7 +877,7 @@ class Instance:
                  material=(384615.38461538462, 0.3, 1000.0, 4e6), cell_schedule=None, numbering="cellwise"):
         import numpy as np
         import torch
-        self.kind, self.degree = kind, degree
+        self.kind, self.degree, self.conductivity = kind, degree, conductivity
         counts = [list(n_cells)]
         while all(c % 2 == 0 for c in counts[0]):
             counts.insert(0, [c // 2 for c in counts[0]])
