@@ -510,11 +510,16 @@ int64_t mfg_session_n_dofs(const mfg_session_spec *spec);
  * Cell work is cut into tasks of up to cells_per_task cells of one colour of
  * one instance.  Schedules: MFG_SCHEDULE_QUEUE, one persistent launch per
  * application (workgroups take tasks from an atomic ticket; a task waits for
- * its instance's previous colour), or MFG_SCHEDULE_COLOUR, one launch per
- * colour.  The tuner times both and the cells per task of every
- * (kind, degree) present, and keeps the fastest. */
+ * its instance's previous colour), MFG_SCHEDULE_COLOUR, one launch per
+ * colour, or MFG_SCHEDULE_STREAMS, every instance's own operator launches
+ * spread over several streams (instances assigned by work, longest first), so
+ * that small instances overlap.  The tuner times the three and the cells per
+ * task of every (kind, degree) present, and keeps the fastest.  For the
+ * batched solver, STREAMS runs every instance's own mfg_cg_solve /
+ * mfg_mg_vmult on worker threads with a stream each. */
 #define MFG_SCHEDULE_COLOUR 0
 #define MFG_SCHEDULE_QUEUE 1
+#define MFG_SCHEDULE_STREAMS 2
 #define MFG_BATCH_SOLID_F1 0 /* tensor2, formulation 1 */
 #define MFG_BATCH_SOLID_F0 1 /* tensor2, formulation 0 */
 #define MFG_BATCH_HEAT 2
@@ -524,6 +529,8 @@ typedef struct mfg_batch_tuning {
   int32_t cells_per_task[4][7]; /* [kind][degree] */
   int32_t waves_per_simd;       /* register budget of the kernel (2 or 4) */
   double ms_colour, ms_queue;   /* one application of the whole batch */
+  double ms_streams;            /* the same with MFG_SCHEDULE_STREAMS (batched solver: one
+                                   preconditioner application of all instances) */
 } mfg_batch_tuning;
 typedef struct mfg_batch_t *mfg_batch;
 /* All operators must be cached and share one precision. */

@@ -9,12 +9,15 @@
 // sequence of its own MGData::vmult / cg_solve, so batched results equal the
 // per-instance results.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batched_kernels.hpp"
@@ -35,9 +38,9 @@ static int max_cpb(int p) { return kBatchThreads / ((p + 1) * (p + 1) * (p + 1))
 
 struct Tuning {
   int cpb[8 * kNumKinds] = {};  // per code; 0 = the largest that fits kBatchThreads
-  int queue = 1;
+  int queue = 1;     // schedule: MFG_SCHEDULE_COLOUR, _QUEUE or _STREAMS
   int waves = 2;     // register budget of the kernel: 2 or 4 waves per SIMD (FP32 only)
-  double ms_colour = 0, ms_queue = 0;
+  double ms_colour = 0, ms_queue = 0, ms_streams = 0;
   int cpb_for(int code) const {
     const int m = max_cpb(code & 7), c = cpb[code];
     return c > 0 && c <= m ? c : m;
@@ -69,6 +72,66 @@ static int queue_capacity() {
   }();
   return cap;
 }
+
+// Streams of the STREAMS schedule: instances are spread over them by work
+// (longest first onto the least-loaded stream), each running its own
+// operator's launches, so small instances overlap instead of queueing behind
+// each other.  Forked from and joined back to the caller's stream with
+// events (which also works under graph capture).  Default priority.
+struct StreamPool {
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<std::vector<int>> assign;  // instances per stream
+  StreamPool() = default;
+  StreamPool(const StreamPool &) = delete;
+  StreamPool &operator=(const StreamPool &) = delete;
+  ~StreamPool() {
+    for (hipStream_t x : streams) (void)hipStreamDestroy(x);
+    for (hipEvent_t e : done) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+  }
+  void build(const std::vector<double> &work, int max_streams) {
+    const int n = (int)work.size();
+    const int S = std::max(1, std::min(max_streams, n));
+    while ((int)streams.size() < S) {
+      hipStream_t x;
+      hipEvent_t e;
+      MFG_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+      streams.push_back(x);
+      MFG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      done.push_back(e);
+    }
+    if (!fork_ev) MFG_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<double> load(S, 0.0);
+    assign.assign(S, {});
+    for (int i : order) {
+      const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[j] += work[i];
+      assign[j].push_back(i);
+    }
+  }
+  // run(i, stream) for every instance on its stream, ordered after the
+  // caller's work and before the caller's next
+  template <class F>
+  void run(hipStream_t caller, F &&f) {
+    MFG_HIP(hipEventRecord(fork_ev, caller));
+    for (size_t j = 0; j < assign.size(); ++j) {
+      if (assign[j].empty()) continue;
+      MFG_HIP(hipStreamWaitEvent(streams[j], fork_ev, 0));
+      for (int i : assign[j]) f(i, streams[j]);
+      MFG_HIP(hipEventRecord(done[j], streams[j]));
+      MFG_HIP(hipStreamWaitEvent(caller, done[j], 0));
+    }
+  }
+};
+
+// work estimate of an operator application: quadrature points times the
+// words read per point
+static double apply_work(const OperatorData &op) { return (double)op.nqp * (op.n_fields + 2.0 * op.nc + 1.0); }
 
 // ------------------------------------------------------- apply groups ---
 // Task tables over a set of instances of one precision: one instance table,
@@ -249,16 +312,17 @@ struct ApplyGroup {
   // kernel launches of one application
   int launches(const Tuning &tu) const {
     int l = 0;
-    for (const KindTasks &kt : kinds) l += tu.queue ? 1 : (int)kt.colour_tasks.size() - 1;
+    for (const KindTasks &kt : kinds) l += tu.queue == MFG_SCHEDULE_QUEUE ? 1 : (int)kt.colour_tasks.size() - 1;
     return l;
   }
 
   // vmult (fix_mode 1, first-touch stores) or vmult_add (fix_mode 2, adds)
   void launch(hipStream_t s, const void *vecs, const int *active, bool add, const Tuning &tu, int *error) const {
     const int fix = add ? 2 : 1, assign = add ? 0 : 1;
-    if (prec == MFG_FP64) launch_t<double, 2>(s, vecs, active, fix, assign, tu.queue, error);
-    else if (tu.waves == 4) launch_t<float, 4>(s, vecs, active, fix, assign, tu.queue, error);
-    else launch_t<float, 2>(s, vecs, active, fix, assign, tu.queue, error);
+    const int queue = tu.queue == MFG_SCHEDULE_QUEUE ? 1 : 0;
+    if (prec == MFG_FP64) launch_t<double, 2>(s, vecs, active, fix, assign, queue, error);
+    else if (tu.waves == 4) launch_t<float, 4>(s, vecs, active, fix, assign, queue, error);
+    else launch_t<float, 2>(s, vecs, active, fix, assign, queue, error);
   }
 
   // Table of (src, dst) per entry, in entry order.
@@ -566,9 +630,48 @@ struct BatchVmult {
     last_src = s;
     last_dst = d;
   }
+  StreamPool pool;
+  // STREAMS: every instance's own apply (its default cell schedule) on the
+  // pool's streams; bitwise the per-instance result
+  void apply_streams(const void *const *src, void *const *dst, bool add) {
+    const size_t n = group->ops.size();
+    for (size_t i = 0; i < n; ++i) {
+      MFG_REQUIRE(src[i] && dst[i], MFG_ERR_ARGUMENT, "null vector");
+      MFG_REQUIRE(src[i] != dst[i], MFG_ERR_ARGUMENT, "vmult needs distinct source and destination");
+    }
+    if (pool.assign.empty()) {
+      std::vector<double> w;
+      for (const OperatorData *op : group->ops) w.push_back(apply_work(*op));
+      pool.build(w, kBatchStreams);
+    }
+    pool.run(stream, [&](int i, hipStream_t s) { group->ops[i]->apply(src[i], dst[i], add, s); });
+  }
   void apply(const void *const *src, void *const *dst, bool add) {
+    if (tu.queue == MFG_SCHEDULE_STREAMS) {
+      apply_streams(src, dst, add);
+      return;
+    }
     set_vectors(src, dst);
     group->launch(stream, vecs.ptr, nullptr, add, tu, error.as<int>());
+  }
+  // the fused schedules' tuning, then the STREAMS schedule timed against the best of them
+  void tune(const void *const *src, void *const *dst, int reps) {
+    set_vectors(src, dst);
+    group->tune(tu, vecs.ptr, reps, stream, error.as<int>());
+    hipEvent_t e0, e1;
+    MFG_HIP(hipEventCreate(&e0));
+    MFG_HIP(hipEventCreate(&e1));
+    for (int w = 0; w < 2; ++w) apply_streams(src, dst, false);
+    MFG_HIP(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; ++r) apply_streams(src, dst, false);
+    MFG_HIP(hipEventRecord(e1, stream));
+    MFG_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MFG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tu.ms_streams = (double)ms / reps;
+    if (tu.ms_streams < std::min(tu.ms_colour, tu.ms_queue)) tu.queue = MFG_SCHEDULE_STREAMS;
   }
   void synchronize() {
     MFG_HIP(hipStreamSynchronize(stream));
@@ -586,6 +689,11 @@ struct BatchSolver {
   hipStream_t stream = nullptr;
   Tuning tu_fine, tu_mg;
   bool use_graph = true;
+  // STREAMS schedule: every instance's own MGData::vmult / cg_solve, the
+  // instances spread over worker threads with a stream each
+  bool concurrent = false;
+  double ms_lockstep = 0, ms_concurrent = 0;  // one preconditioner application of all instances
+  StreamPool pool;
 
   std::unique_ptr<ApplyGroup> fine_group;
   std::map<std::string, std::unique_ptr<ApplyGroup>> level_groups;
@@ -987,10 +1095,40 @@ struct BatchSolver {
     reset_programs();
     MFG_HIP(hipStreamSynchronize(stream));
     check_error();
+    // then the schedule of whole preconditioner applications: the lockstep
+    // program (graph replays) against every instance's own V-cycle on the
+    // worker threads, on the workspace vectors r -> z
+    std::vector<const void *> r(n);
+    std::vector<void *> z(n);
+    for (int i = 0; i < n; ++i) {
+      r[i] = slice(ws_r, i);
+      z[i] = slice(ws_z, i);
+    }
+    auto time_pc = [&](bool conc) {
+      concurrent = conc;
+      for (int w = 0; w < 3; ++w) precondition(r.data(), z.data());
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < reps; ++k) precondition(r.data(), z.data());
+      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+    };
+    ms_lockstep = time_pc(false);
+    ms_concurrent = time_pc(true);
+    concurrent = ms_concurrent < ms_lockstep;
+    const int sched = concurrent ? MFG_SCHEDULE_STREAMS : (tu_mg.queue == MFG_SCHEDULE_QUEUE ? 1 : 0);
+    tu_mg.ms_streams = tu_fine.ms_streams = ms_concurrent;
+    if (concurrent) tu_fine.queue = tu_mg.queue = sched;
   }
 
   // z = MG(r) for every instance
   void precondition(const void *const *src, void *const *dst) {
+    if (concurrent) {
+      std::string err;
+      std::vector<int32_t> st(n, MFG_OK);
+      run_concurrent([&](int i) { mg[i]->vmult(dst[i], src[i], fprec()); }, st.data(), err);
+      for (int i = 0; i < n; ++i)
+        if (st[i] != MFG_OK) throw Error(st[i], "batched preconditioner: " + err);
+      return;
+    }
     for (int i = 0; i < n; ++i) host_active[i] = 1;
     upload_active();
     std::vector<VecSeg> in, out;
@@ -1009,9 +1147,96 @@ struct BatchSolver {
     check_error();
   }
 
+  // Runs f(i) for every instance on the pool's worker threads, instance i's
+  // operators (fine and levels) switched to its worker's stream meanwhile.
+  // Errors of one instance go to status[i] (when given) and stop nothing else.
+  void ensure_pool() {
+    if (!pool.assign.empty()) return;
+    std::vector<double> w;
+    for (int i = 0; i < n; ++i) {
+      double x = apply_work(*fine[i]);
+      for (MGLevel &L : mg[i]->levels) x += 11.0 * apply_work(*L.op);  // operator applications per cycle
+      w.push_back(x);
+    }
+    pool.build(w, kBatchStreams);
+  }
+  template <class F>
+  void run_concurrent(F &&f, int32_t *status, std::string &first_error) {
+    ensure_pool();
+    MFG_HIP(hipDeviceSynchronize());  // the callers' vectors are complete on every stream
+    std::mutex mu;
+    std::vector<std::thread> workers;
+    for (size_t j = 0; j < pool.assign.size(); ++j) {
+      workers.emplace_back([&, j] {
+        hipStream_t st = pool.streams[j];
+        for (int i : pool.assign[j]) {
+          std::vector<OperatorData *> ops{fine[i]};
+          for (MGLevel &L : mg[i]->levels)
+            if (L.op != fine[i]) ops.push_back(L.op);
+          std::vector<hipStream_t> saved;
+          for (OperatorData *op : ops) {
+            saved.push_back(op->stream);
+            op->stream = st;
+          }
+          int code = MFG_OK;
+          std::string msg;
+          try {
+            f(i);
+            MFG_HIP(hipStreamSynchronize(st));
+          } catch (const Error &e) {
+            code = e.code;
+            msg = e.what();
+            (void)hipStreamSynchronize(st);
+          } catch (const std::exception &e) {
+            code = MFG_ERR_ARGUMENT;
+            msg = e.what();
+            (void)hipStreamSynchronize(st);
+          }
+          for (size_t k = 0; k < ops.size(); ++k) ops[k]->stream = saved[k];
+          if (status) status[i] = code;
+          if (code != MFG_OK) {
+            std::lock_guard<std::mutex> g(mu);
+            if (first_error.empty()) first_error = msg;
+          }
+        }
+      });
+    }
+    for (auto &t : workers) t.join();
+  }
+
+  // every instance's own cg_solve (MFG_PRECOND_MG), concurrently
+  void solve_concurrent(const void *const *b, void *const *x, double tol_rel, int max_iter, mfg_solve_info *info,
+                        int32_t *status) {
+    ensure_pool();
+    // one CG workspace and reducer per worker
+    std::vector<std::unique_ptr<CGWorkspace>> ws(pool.assign.size());
+    std::vector<std::unique_ptr<Reducer>> red(pool.assign.size());
+    std::vector<int> worker_of(n, 0);
+    for (size_t j = 0; j < pool.assign.size(); ++j) {
+      ws[j].reset(new CGWorkspace());
+      red[j].reset(new Reducer());
+      for (int i : pool.assign[j]) worker_of[i] = (int)j;
+    }
+    std::string err;
+    run_concurrent(
+        [&](int i) {
+          const int j = worker_of[i];
+          std::memset(&info[i], 0, sizeof(info[i]));
+          cg_solve(*fine[i], MFG_PRECOND_MG, mg[i], 1.0, b[i], x[i], tol_rel, max_iter, &info[i], *ws[j], *red[j]);
+        },
+        status, err);
+    for (int i = 0; i < n; ++i)
+      if (status[i] != MFG_OK && status[i] != MFG_ERR_NOCONV && status[i] != MFG_ERR_NUMERIC)
+        throw Error(status[i], "batched solve: " + err);
+  }
+
   // cg_solve with the multigrid preconditioner, all instances in lockstep
   void solve(const void *const *b, void *const *x, double tol_rel, int max_iter, mfg_solve_info *info,
              int32_t *status) {
+    if (concurrent) {
+      solve_concurrent(b, x, tol_rel, max_iter, info, status);
+      return;
+    }
     for (int i = 0; i < n; ++i) {
       MFG_REQUIRE(b[i] && x[i], MFG_ERR_ARGUMENT, "null vector");
       host_active[i] = 1;
@@ -1141,12 +1366,13 @@ struct mfg_batch_solver_t {
 static void export_tuning(const Tuning &tu, mfg_batch_tuning *out) {
   if (!out) return;
   std::memset(out, 0, sizeof(*out));
-  out->schedule = tu.queue ? MFG_SCHEDULE_QUEUE : MFG_SCHEDULE_COLOUR;
+  out->schedule = tu.queue;
   for (int kind = 0; kind < kNumKinds; ++kind)
     for (int p = 1; p <= kMaxDegree; ++p) out->cells_per_task[kind][p] = tu.cpb_for(kind * 8 + p);
   out->waves_per_simd = tu.waves;
   out->ms_colour = tu.ms_colour;
   out->ms_queue = tu.ms_queue;
+  out->ms_streams = tu.ms_streams;
 }
 
 extern "C" {
@@ -1179,9 +1405,10 @@ int mfg_batch_set_stream(mfg_batch b, void *stream) {
 int mfg_batch_set_schedule(mfg_batch b, int32_t schedule) {
   return guarded([&] {
     MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
-    MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE, MFG_ERR_ARGUMENT,
-                "unknown schedule");
-    b->b->tu.queue = schedule == MFG_SCHEDULE_QUEUE;
+    MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE ||
+                    schedule == MFG_SCHEDULE_STREAMS,
+                MFG_ERR_ARGUMENT, "unknown schedule");
+    b->b->tu.queue = schedule;
   });
 }
 
@@ -1222,8 +1449,7 @@ int mfg_batch_tune(mfg_batch b, const void *const *src, void *const *dst, int32_
   return guarded([&] {
     MFG_REQUIRE(b && src && dst && repetitions > 0, MFG_ERR_ARGUMENT, "null argument");
     BatchVmult &v = *b->b;
-    v.set_vectors(src, dst);
-    v.group->tune(v.tu, v.vecs.ptr, repetitions, v.stream, v.error.as<int>());
+    v.tune(src, dst, repetitions);
     v.synchronize();
     export_tuning(v.tu, result);
   });
@@ -1276,10 +1502,16 @@ int mfg_batch_solver_tune(mfg_batch_solver s, int32_t repetitions, mfg_batch_tun
 int mfg_batch_solver_set_schedule(mfg_batch_solver s, int32_t schedule) {
   return guarded([&] {
     MFG_REQUIRE(s, MFG_ERR_ARGUMENT, "null solver");
-    MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE, MFG_ERR_ARGUMENT,
-                "unknown schedule");
-    s->s->tu_fine.queue = s->s->tu_mg.queue = schedule == MFG_SCHEDULE_QUEUE;
-    s->s->build();
+    MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE ||
+                    schedule == MFG_SCHEDULE_STREAMS,
+                MFG_ERR_ARGUMENT, "unknown schedule");
+    s->s->concurrent = schedule == MFG_SCHEDULE_STREAMS;
+    if (!s->s->concurrent) {
+      s->s->tu_fine.queue = s->s->tu_mg.queue = schedule;
+      s->s->build();
+    } else {
+      s->s->tu_fine.queue = s->s->tu_mg.queue = schedule;
+    }
   });
 }
 

@@ -29,6 +29,7 @@
 namespace mfg {
 
 constexpr int kBatchThreads = 512;
+constexpr int kBatchStreams = 8;  // streams of the STREAMS schedule (and worker threads of its solves)
 constexpr int kSpinLimit = 1 << 22;
 
 // instance kinds in the dispatch code (code = kind * 8 + p); kSolidF1G is

@@ -58,7 +58,7 @@ static void fill(Number *x, int64_t n, Number v, hipStream_t s) {
 
 // --------------------------------------------------------- dispatchers ---
 template <int P, typename Number>
-static void launch_apply(const OperatorData &op, const void *src, void *dst, int fix_mode) {
+static void launch_apply(const OperatorData &op, const void *src, void *dst, int fix_mode, hipStream_t stream) {
   constexpr int CPB = DefaultCPB<P, Number>::value;
   ApplyArgs<Number> a;
   a.cell_nodes = op.cell_nodes.as<uint32_t>();
@@ -104,7 +104,7 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
         b.seg_end = op.bundle_colour_start[col + 1];
         b.fix_mode = col == 0 ? fix_mode : 0;
         if (b.seg_end == b.seg_begin) continue;
-        kernel<<<b.seg_end - b.seg_begin, bundle_threads<P, BW>(), 0, op.stream>>>(b, q);
+        kernel<<<b.seg_end - b.seg_begin, bundle_threads<P, BW>(), 0, stream>>>(b, q);
         MFG_HIP(hipGetLastError());
       }
       return true;
@@ -124,7 +124,7 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
       a.fix_mode = col == 0 ? fix_mode : 0;
       const int groups = blocks_for(a.cell_end - a.cell_begin, CPB);
       if (groups == 0) continue;
-      kernel<<<groups, NT, 0, op.stream>>>(a, q, assign);
+      kernel<<<groups, NT, 0, stream>>>(a, q, assign);
       MFG_HIP(hipGetLastError());
     }
   };
@@ -136,10 +136,10 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
     a.cell_begin = 0;
     a.cell_end = op.n_cells;
     a.fix_mode = 0;
-    kernel<<<blocks_for(op.n_cells, CPB), NT, 0, op.stream>>>(a, q, op.contrib.as<Number>());
+    kernel<<<blocks_for(op.n_cells, CPB), NT, 0, stream>>>(a, q, op.contrib.as<Number>());
     MFG_HIP(hipGetLastError());
     const int64_t nn = op.n_nodes;
-    gather_contributions<Number, NC><<<(int)((nn + 255) / 256), 256, 0, op.stream>>>(
+    gather_contributions<Number, NC><<<(int)((nn + 255) / 256), 256, 0, stream>>>(
         op.contrib.as<Number>(), op.gather_offsets.as<int32_t>(), op.gather_points.as<int32_t>(),
         op.gather_cmask.as<uint8_t>(), (const Number *)src, (Number *)dst, nn, fix_mode);
     MFG_HIP(hipGetLastError());
@@ -259,8 +259,8 @@ static void launch_diagonal(OperatorData &op) {
   }
 
 template <typename Number>
-static void apply_t(const OperatorData &op, const void *src, void *dst, int fix_mode) {
-  MFG_DISPATCH_DEGREE(op.p, launch_apply, op, src, dst, fix_mode);
+static void apply_t(const OperatorData &op, const void *src, void *dst, int fix_mode, hipStream_t stream) {
+  MFG_DISPATCH_DEGREE(op.p, launch_apply, op, src, dst, fix_mode, stream);
 }
 template <typename Number>
 static void cache_t(OperatorData &op, const void *u, int u_fp32) {
@@ -271,23 +271,24 @@ static void diagonal_t(OperatorData &op) {
   MFG_DISPATCH_DEGREE(op.p, launch_diagonal, op);
 }
 
-void OperatorData::apply(const void *src, void *dst, bool add) const {
+void OperatorData::apply(const void *src, void *dst, bool add, hipStream_t on) const {
   MFG_REQUIRE(ready, MFG_ERR_STATE, "operator used before cache()");
   MFG_REQUIRE(src && dst, MFG_ERR_ARGUMENT, "null vector");
   // the scatter reads dst while other workgroups still gather src
   MFG_REQUIRE(src != dst, MFG_ERR_ARGUMENT, "dst and src must be different vectors");
+  const hipStream_t s = on ? on : stream;
   // vmult needs no zero fill: the first colour touching a node stores
-  timing_begin();
-  if (precision == MFG_FP64) apply_t<double>(*this, src, dst, add ? 2 : 1);
-  else apply_t<float>(*this, src, dst, add ? 2 : 1);
-  timing_end();
+  timing_begin(s);
+  if (precision == MFG_FP64) apply_t<double>(*this, src, dst, add ? 2 : 1, s);
+  else apply_t<float>(*this, src, dst, add ? 2 : 1, s);
+  timing_end(s);
 }
 
 // Event pairs are reused; at most kMaxEvents launches are kept between
 // queries (older ones are folded in by an early collect).
 static constexpr size_t kMaxEvents = 2048;
 
-void OperatorData::timing_begin() const {
+void OperatorData::timing_begin(hipStream_t s) const {
   if (!timing) return;
   if (events_used == events.size()) {
     if (events.size() >= kMaxEvents) throw Error(MFG_ERR_STATE, "too many timed launches between queries");
@@ -296,12 +297,12 @@ void OperatorData::timing_begin() const {
     MFG_HIP(hipEventCreate(&b));
     events.emplace_back(a, b);
   }
-  MFG_HIP(hipEventRecord(events[events_used].first, stream));
+  MFG_HIP(hipEventRecord(events[events_used].first, s));
 }
 
-void OperatorData::timing_end() const {
+void OperatorData::timing_end(hipStream_t s) const {
   if (!timing) return;
-  MFG_HIP(hipEventRecord(events[events_used].second, stream));
+  MFG_HIP(hipEventRecord(events[events_used].second, s));
   ++events_used;
 }
 

@@ -73,12 +73,13 @@ struct OperatorData {
   void setup_solid();
   void setup_heat(const mfg_level_desc &d, const double *coef);
   void cache(const void *u, int u_precision);
-  void apply(const void *src, void *dst, bool add) const;
+  // dst (+)= A src on `on` (the operator's stream when null)
+  void apply(const void *src, void *dst, bool add, hipStream_t on = nullptr) const;
   void compute_diagonal();
   size_t memory_consumption() const;
   size_t elem_size() const { return precision == MFG_FP64 ? 8 : 4; }
-  void timing_begin() const;
-  void timing_end() const;
+  void timing_begin(hipStream_t s) const;
+  void timing_end(hipStream_t s) const;
   void timing_collect(double &ms, int64_t &n);
 };
 

@@ -45,7 +45,7 @@ SPECS = [("heat", [3, 4, 2], 1, 1), ("solid", [2, 3, 2], 2, 1), ("heat", [4, 2, 
 
 @pytest.mark.parametrize("precision,storage", [(mfgpu.FP64, mfgpu.T2_FOLDED), (mfgpu.FP32, mfgpu.T2_FOLDED),
                                                (mfgpu.FP64, mfgpu.T2_GRADIENT)])
-@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE])
+@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE, mfgpu.SCHEDULE_STREAMS])
 def test_batched_vmult_equals_per_instance(cuda, precision, storage, schedule):
     print(f"batched vmult precision={precision} storage={storage} schedule={schedule}", flush=True)
     ops, meshes = make_ops(SPECS, precision, storage)
@@ -91,7 +91,8 @@ def test_batched_cells_per_task_and_tuner(cuda):
     t = b.tune(srcs, out, repetitions=2)
     d = t.as_dict()
     print("tuning", d)
-    assert d["schedule"] in ("queue", "colour") and t.waves_per_simd in (2, 4)
+    assert d["schedule"] in ("queue", "colour", "streams") and t.waves_per_simd in (2, 4)
+    assert t.ms_streams > 0
     for p in range(1, 7):
         assert 1 <= d["cells_per_task"]["heat"][p - 1] <= 512 // (p + 1) ** 3
     out = [torch.empty_like(s) for s in srcs]
@@ -113,7 +114,7 @@ ISPECS = [("solid", [4, 2, 2], 2), ("heat", [4, 4, 4], 2), ("solid", [3, 2, 2], 
 
 
 @pytest.mark.parametrize("graph", [True, False])
-@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE])
+@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE, mfgpu.SCHEDULE_STREAMS])
 def test_batched_vcycle_equals_per_instance(cuda, graph, schedule):
     print(f"batched V-cycle graph={graph} schedule={schedule}", flush=True)
     inst = instances(ISPECS)
@@ -133,12 +134,16 @@ def test_batched_vcycle_equals_per_instance(cuda, graph, schedule):
             assert torch.equal(o, r), (rep, k, ISPECS[k])
         torch.cuda.synchronize()
         _ = sum(float(s.sum()) for s in srcs)
+    if schedule == mfgpu.SCHEDULE_STREAMS:  # every instance's own V-cycle: no batched program
+        return
     launches, nodes = bs.info()
     print(f"V-cycle program: {launches} launches, graph nodes {nodes}")
     assert launches > 0 and (nodes > 0 if graph else True)
 
 
-def test_batched_cg_equals_per_instance(cuda):
+@pytest.mark.parametrize("schedule", [None, mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_STREAMS])
+def test_batched_cg_equals_per_instance(cuda, schedule):
+    """None: the tuner's choice."""
     inst = instances(ISPECS)
     bs_ = []
     for k, i in enumerate(inst):
@@ -155,6 +160,8 @@ def test_batched_cg_equals_per_instance(cuda):
     solver = mfgpu.BatchSolver([i.fine for i in inst], [i.mg for i in inst])
     fine_t, level_t = solver.tune(repetitions=2)
     print("tuned fine", fine_t.as_dict(), "levels", level_t.as_dict())
+    if schedule is not None:
+        solver.set_schedule(schedule)
     xs = [torch.zeros_like(b) for b in bs_]
     infos, status = solver.solve(bs_, xs, tol_rel=1e-8)
     for k in range(len(inst)):
