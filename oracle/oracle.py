@@ -234,6 +234,7 @@ class Solid:
     def __init__(self, mesh, n_q=None, mu=MU, nu=NU, rho=RHO, alpha1=ALPHA1, formulation=1, caching=1):
         self.mesh = mesh
         n_q = mesh.p + 1 if n_q is None else n_q
+        self.mu, self.nu, self.rho, self.alpha1 = mu, nu, rho, alpha1
         self.h = lib().orc_solid_create(mesh.h, n_q, mu, nu, rho, alpha1, formulation, caching)
         if not self.h:
             raise ValueError("unsupported solid operator configuration")
@@ -416,6 +417,64 @@ def cheb_dense(A, b, degree, smoothing_range, eig_it, x0=None):
     lib().orc_cheb_dense(n, dptr(A), degree, smoothing_range, eig_it, 1 if x0 is None else 0, dptr(b), dptr(x),
                          dptr(info))
     return x, dict(lmax=info[0], lmin=info[1], theta=info[2], delta=info[3], degree=int(info[4]))
+
+
+def _simd_lib():
+    """The cross-cell SIMD baseline library for this host: AVX-512 if the CPU
+    has it, else AVX2."""
+    global _SIMD
+    if _SIMD is None:
+        flags = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                flags = next((l for l in f if l.startswith("flags")), "")
+        except OSError:
+            pass
+        isa = "avx512" if " avx512f" in flags else "avx2"
+        L = ctypes.CDLL(os.path.join(_HERE, f"liboracle_simd_{isa}.so"))
+        L.orc_simd_solid_create.restype = _vp
+        L.orc_simd_solid_create.argtypes = [ctypes.c_int, _ip, ctypes.c_int64, _ip, ctypes.c_int64, _ip, _dp,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_simd_solid_destroy.argtypes = [_vp]
+        L.orc_simd_solid_vmult.argtypes = [_vp, _dp, _dp]
+        L.orc_simd_lanes.restype = ctypes.c_int
+        _SIMD = (L, isa)
+    return _SIMD
+
+
+_SIMD = None
+
+
+class SimdSolid:
+    """The tangent vmult of a cached tensor2 formulation-1 Solid (degree 2 or
+    4) computed the way the reference's CPU path computes it: batches of 8
+    cells in SIMD lanes, OpenMP over the batches of a colour (the CPU
+    baseline of the benchmark; equals Solid.vmult)."""
+
+    def __init__(self, solid):
+        L, self.isa = _simd_lib()
+        self.lanes = L.orc_simd_lanes()
+        m = solid.mesh
+        f = solid.tensor2_data()
+        n = np.ascontiguousarray(m.n, dtype=np.int32)
+        cn = np.ascontiguousarray(m.cell_nodes(), dtype=np.int32)
+        con = np.ascontiguousarray(m.constrained(), dtype=np.int32)
+        self.n_dofs = m.n_dofs
+        self.h = L.orc_simd_solid_create(m.p, iptr(n), m.n_dofs, iptr(cn), len(con), iptr(con),
+                                         dptr(np.ascontiguousarray(f)), solid.mu, solid.nu, solid.rho * solid.alpha1)
+        if not self.h:
+            raise ValueError("the SIMD baseline covers degrees 2 and 4")
+        self._L = L
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.orc_simd_solid_destroy(self.h)
+            self.h = None
+
+    def vmult(self, src, dst=None):
+        dst = np.empty(self.n_dofs) if dst is None else dst
+        self._L.orc_simd_solid_vmult(self.h, dptr(dst), dptr(np.ascontiguousarray(src)))
+        return dst
 
 
 def box_levels(lo, hi, coarse_n, n_levels, p, nc, face_mask=1, comp_mask=7, face_comp=None):

@@ -280,7 +280,7 @@ def test_pins_fail_on_a_planted_mistake(plant, caught_by, tmp_path):
     here = os.path.dirname(os.path.abspath(__file__))
     odir = os.path.join(here, "..", "oracle")
     lib = str(tmp_path / "liboracle_planted.so")
-    r = subprocess.run(["make", "-s", "-C", odir, "-B", f"LIB={lib}", f"PLANT=-D{plant}"], capture_output=True,
+    r = subprocess.run(["make", "-s", "-C", odir, "-B", f"LIB={lib}", f"PLANT=-D{plant}", lib], capture_output=True,
                        text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     env = dict(os.environ, MFG_ORACLE_LIB=lib)
