@@ -27,12 +27,31 @@ BEAM = dict(lo=[0.0, 0.0, 0.0], hi=[4.0, 1.0, 1.0], cells=[128, 32, 32], degree=
 # node numbering of every bench mesh (the vector layout in HBM); "cellwise" gives each cell a contiguous block
 NUMBERING = "cellwise"
 
-# HBM bytes per vmult of the current cell kernel from two rocprofv3 --pmc passes (FETCH_SIZE x 2.00 and
-# WRITE_SIZE x 1.00, factors from a copy of known size); PMC counters cannot be read from inside this
-# process, so the figure is re-measured whenever the kernel changes.
-MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise", "gradient"): 2663564902,
-                    ((128, 32, 32), 4, "cellwise", "folded"): 3554406400,
-                    ((128, 32, 32), 4, "lexicographic", "folded"): 4017419059}
+# HBM bytes per vmult of the cell kernel from two rocprofv3 --pmc passes (FETCH_SIZE x 2.00 and WRITE_SIZE
+# x 1.00, factors from a copy of known size), with the SHA-256 prefix of the kernel source they were measured
+# on.  PMC counters cannot be read from inside this process: when the source differs, the figure no longer
+# describes the kernel and the bench reports no traffic until it is re-measured.
+TRAFFIC_SOURCE = "profiles/vmult_q4_beam_fp64_pmc_traffic.md"
+KERNEL_SOURCE = os.path.join(ROOT, "matrix-free-dealii-precice_amd", "csrc", "cell_kernels.hpp")
+MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise", "gradient"): (2663564902, "002782b0372e"),
+                    ((128, 32, 32), 4, "cellwise", "folded"): (3554406400, "002782b0372e")}
+
+
+def kernel_source_hash():
+    import hashlib
+    with open(KERNEL_SOURCE, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:12]
+
+
+def measured_traffic(key):
+    """(bytes, note) for the bench's operator, bytes None when the kernel changed since the measurement."""
+    if key not in MEASURED_TRAFFIC:
+        return None, "not measured for this configuration"
+    b, h = MEASURED_TRAFFIC[key]
+    now = kernel_source_hash()
+    if h != now:
+        return None, f"the kernel source ({now}) differs from the profiled one ({h}): re-measure"
+    return b, f"{TRAFFIC_SOURCE}, kernel source sha256 {h}"
 
 
 def log(msg):
@@ -436,7 +455,10 @@ def heat_bench(mfgpu, torch, numbering, n=(96, 96, 96), p=4, reps=30):
                     "cache_words_per_qp": op.n_fields(), "canonical_bytes": canonical,
                     "canonical_gbs": round(canonical / (ms * 1e-3) / 1e9, 1),
                     "canonical_frac": round(canonical / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "layout_bytes": layout}
+                    "layout_bytes": layout,
+                    "layout_frac": round(layout / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "layout_note": "the bytes the layout holds (c per point, J^-1 and det J per cell on affine "
+                                   "meshes): the actual-bytes counterpart of canonical_frac"}
         log(f"heat {key}: {ms:.3f} ms")
         del op, src, dst
         torch.cuda.empty_cache()
@@ -537,6 +559,7 @@ def main():
     n_qp_all = n_cell * (p + 1) ** 3
     achieved = B / (kern_avg_ms * 1e-3) / 1e9
 
+    traffic = measured_traffic((tuple(args.cells), p, args.numbering, args.point_storage))
     # per-vmult latency distribution (events around each call)
     lat = []
     for _ in range(min(args.steps, 50)):
@@ -566,8 +589,8 @@ def main():
                    "point_storage": args.point_storage, "words_per_point": op.n_fields()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((tuple(args.cells), p, args.numbering, args.point_storage)),
-                     "traffic_source": "profiles/vmult_q4_beam_fp64_pmc_traffic.md",
+                     "traffic": traffic[0],
+                     "traffic_source": traffic[1],
                      "kernel": f"apply_cells<{p},1,double,{'QSolidT2F1G' if gradient_storage else 'QSolidT2F1'}>",
                      "kernel_avg_ms": round(kern_avg_ms, 4),
                      "canonical_bytes_per_launch": B,
