@@ -252,21 +252,13 @@ int32_t mfg_operator_n_colours(mfg_operator op);
 
 /* Cell schedule of vmult.  MFG_CELLS_COLOURS (the default above 2^20
  * quadrature points): one launch per cell colour, a workgroup per group of
- * cells.  MFG_CELLS_BUNDLES
- * (structured boxes only): a workgroup walks a segment of 2x2-cell bundles
- * along x, summing shared faces in LDS and carrying the x face to the next
- * step, so most nodes are gathered and stored once; MFG_CELLS_PENCILS: the
- * same with one cell per step.  All give the same operator (rounding aside).
- * A mesh that is not a structured box, or a kernel that would not fit the GPU
- * without spilling, keeps (or falls back to) COLOURS.  MFG_CELLS_GATHER: one
+ * cells.  MFG_CELLS_GATHER: one
  * launch over all cells storing each point's contribution, then one launch
  * summing every node's contributions in colour order -- bitwise the COLOURS
  * result, in two launches instead of one per colour (the default up to 2^20
  * quadrature points, where colour launches are latency bound); it holds
  * N_qp * n_components extra values. */
 #define MFG_CELLS_COLOURS 0
-#define MFG_CELLS_BUNDLES 1
-#define MFG_CELLS_PENCILS 2 /* as BUNDLES with one cell per step */
 #define MFG_CELLS_GATHER 3
 int mfg_operator_set_cell_schedule(mfg_operator op, int32_t schedule);
 int32_t mfg_operator_cell_schedule(mfg_operator op);

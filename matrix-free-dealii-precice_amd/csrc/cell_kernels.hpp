@@ -955,140 +955,4 @@ __global__ void gather_contributions(const Number *__restrict__ contrib, const i
   }
 }
 
-// ------------------------------------------------------ bundle kernel ---
-// Structured boxes: a workgroup owns a bundle of 2 x 2 cells (in y, z) and
-// walks a segment of consecutive cells along x, one bundle step at a time.
-// Nodes shared inside a step (the y and z faces between the four cells) are
-// summed in LDS by their owner -- the present cell with the lowest (z, y)
-// position -- in a fixed order; the x face between two steps is carried in
-// LDS to the next step.  Most nodes are thus gathered once and stored once,
-// without a read of dst.  Segments are coloured (x parity, y and z bundle
-// parity), one launch per colour, so the remaining shared nodes (segment and
-// bundle boundaries) are a plain read-modify-write as in apply_cells.  Bit 31
-// of the bundle map marks the entry that stores a node first.
-// BW: cells per step, 4 (2 x 2 bundle) or 1 (a pencil of single cells)
-template <int P, int BW = 4>
-constexpr int bundle_threads() {
-  return ((BW * (P + 1) * (P + 1) * (P + 1) + 63) / 64) * 64;
-}
-
-template <typename Number>
-struct BundleArgs {
-  const uint32_t *map;    // cell map, first-touch bits in bundle order
-  const int32_t *cells;   // [segment][steps][BW] stored cell index or -1
-  const int32_t *len;     // [segment] steps
-  int seg_begin, seg_end, steps;
-  const Number *src;
-  Number *dst;
-  const Number *shape, *colloc;
-  const int32_t *constrained;
-  int64_t n_constrained;
-  int fix_mode;
-  int assign;  // vmult: the first store of a node overwrites; vmult_add: always adds
-};
-
-template <int P, typename Number, class QOp, int BW>
-__global__ void __launch_bounds__((bundle_threads<P, BW>()))
-    apply_bundles(BundleArgs<Number> a, QOp qop) {
-  constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC, NT = bundle_threads<P, BW>();
-  constexpr int M = BW == 4 ? 2 * N - 1 : N;  // extent of the carried x plane
-  __shared__ Number s_shape[N * N], s_colloc[N * N];
-  __shared__ Number s_tile[BW * 3 * NC * N3];
-  __shared__ Number s_contrib[(BW == 4 ? 4 : 1) * NC * N3];
-  __shared__ Number s_carry[2 * M * M * NC];
-  const int t = threadIdx.x;
-  if (t < N * N) {
-    s_shape[t] = a.shape[t];
-    s_colloc[t] = a.colloc[t];
-  }
-  if (a.fix_mode) {  // rows of constrained dofs: identity (mf_nh_operator.h:622-624)
-    for (int64_t idx = (int64_t)blockIdx.x * NT + t; idx < a.n_constrained; idx += (int64_t)gridDim.x * NT) {
-      const int64_t dof = a.constrained[idx];
-      if (a.fix_mode == 1) a.dst[dof] = a.src[dof];
-      else a.dst[dof] += a.src[dof];
-    }
-  }
-  const int lc_raw = t / N3;
-  const bool pad = lc_raw >= BW;
-  const int lc = pad ? BW - 1 : lc_raw;
-  const int pt = pad ? 0 : t - lc_raw * N3;
-  const int cy = lc & 1, cz = lc >> 1;
-  Tile<N, Number> T;
-  T.base = s_tile + lc * 3 * NC * N3;
-  T.pt = pt;
-  T.i = pt % N;
-  T.j = (pt / N) % N;
-  T.k = pt / (N * N);
-  T.writer = !pad;
-  const int i = T.i, j = T.j, k = T.k;
-  const bool owner = BW == 1 || !((cy == 1 && j == 0) || (cz == 1 && k == 0));
-  const int plane = ((cy * (N - 1) + j) * M + cz * (N - 1) + k) * NC;
-  const int seg = a.seg_begin + xcd_remap(blockIdx.x, gridDim.x);
-  const int len = a.len[seg];
-  const int32_t *seg_cells = a.cells + (int64_t)seg * a.steps * BW;
-  for (int step = 0; step < len; ++step) {
-    const int cell = pad ? -1 : seg_cells[step * BW + lc];
-    const bool active = cell >= 0;
-    const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
-    const uint32_t entry = active ? a.map[qidx] : 0u;
-    const int64_t node = entry & kNodeMaskDev;
-    const bool deferred = owner && i == N - 1 && step + 1 < len;
-    const bool storing = active && owner && !deferred;
-    const bool first = a.assign && (entry >> 31);
-    Number u[NC], dold[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const bool con = (entry >> (kConstraintShift + c)) & 1u;
-      u[c] = (!active || con) ? Number(0) : a.src[node * NC + c];
-      dold[c] = (!storing || first || con) ? Number(0) : a.dst[node * NC + c];
-    }
-    typename QOp::Data qd = qop.load(qidx, active ? cell : 0, pt);
-    __syncthreads();  // tables visible; the previous step's LDS reads are done
-    if constexpr (QOp::kNodalField) {
-      Number w[3], wg[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) w[c] = active ? qop.u[node * 3 + c] : Number(0);
-      evaluate_g<N, 3, Number>(T, s_shape, s_colloc, w, wg);
-      qop.attach(qd, wg);
-    }
-    Number val[NC], grad[NC][3];
-    evaluate_vg<N, NC, Number>(T, s_shape, s_colloc, u, val, grad);
-    qop.apply(qd, val, grad);
-    Number out[NC];
-    integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
-    if (BW == 4 && active && !owner) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) s_contrib[(lc * NC + c) * N3 + pt] = out[c];
-    }
-    __syncthreads();
-    if (active && owner) {
-      // the other cells of the step that touch this node, in a fixed order
-      if (BW == 4 && cy == 0 && j == N - 1 && seg_cells[step * BW + 1 + 2 * cz] >= 0) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) out[c] += s_contrib[((1 + 2 * cz) * NC + c) * N3 + i + N * (N * k)];
-      }
-      if (BW == 4 && cz == 0 && k == N - 1 && seg_cells[step * BW + cy + 2] >= 0) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) out[c] += s_contrib[((cy + 2) * NC + c) * N3 + i + N * j];
-      }
-      if (BW == 4 && cy == 0 && cz == 0 && j == N - 1 && k == N - 1 && seg_cells[step * BW + 3] >= 0) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) out[c] += s_contrib[(3 * NC + c) * N3 + i];
-      }
-      if (i == 0 && step > 0) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) out[c] += s_carry[((step - 1) & 1) * M * M * NC + plane + c];
-      }
-      if (deferred) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) s_carry[(step & 1) * M * M * NC + plane + c] = out[c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          if (!((entry >> (kConstraintShift + c)) & 1u)) a.dst[node * NC + c] = dold[c] + out[c];
-      }
-    }
-  }
-}
-
 }  // namespace mfg

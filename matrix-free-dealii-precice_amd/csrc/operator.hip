@@ -70,52 +70,6 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
   a.n_constrained = op.n_constrained;
   constexpr int NT = block_threads<P, CPB>();
   const int assign = fix_mode == 1 ? 1 : 0;  // vmult: first touch stores, no zero fill
-  // bundle schedule where the mesh is a structured box and the block fits
-  // the GPU without spilling registers
-  auto launch_width = [&](auto q, auto width_tag) -> bool {
-    using QOp = decltype(q);
-    constexpr int BW = decltype(width_tag)::value;
-    if constexpr (bundle_threads<P, BW>() > 1024) {
-      return false;
-    } else {
-      const auto kernel = apply_bundles<P, Number, QOp, BW>;
-      static const bool fits = [&] {
-        int nb = 0;
-        hipFuncAttributes attr;
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, bundle_threads<P, BW>(), 0) == hipSuccess &&
-               nb >= 1 && hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kernel)) == hipSuccess &&
-               attr.localSizeBytes == 0;
-      }();
-      if (!fits) return false;
-      BundleArgs<Number> b;
-      b.map = op.bundle_map.as<uint32_t>();
-      b.cells = op.bundle_cells.as<int32_t>();
-      b.len = op.bundle_len.as<int32_t>();
-      b.steps = op.bundle_steps;
-      b.src = (const Number *)src;
-      b.dst = (Number *)dst;
-      b.shape = op.tab_shape.as<Number>();
-      b.colloc = op.tab_colloc.as<Number>();
-      b.constrained = op.constrained.as<int32_t>();
-      b.n_constrained = op.n_constrained;
-      b.assign = assign;
-      for (size_t col = 0; col + 1 < op.bundle_colour_start.size(); ++col) {
-        b.seg_begin = op.bundle_colour_start[col];
-        b.seg_end = op.bundle_colour_start[col + 1];
-        b.fix_mode = col == 0 ? fix_mode : 0;
-        if (b.seg_end == b.seg_begin) continue;
-        kernel<<<b.seg_end - b.seg_begin, bundle_threads<P, BW>(), 0, stream>>>(b, q);
-        MFG_HIP(hipGetLastError());
-      }
-      return true;
-    }
-  };
-  auto launch_bundles = [&](auto q) -> bool {
-    if ((op.cell_schedule != MFG_CELLS_BUNDLES && op.cell_schedule != MFG_CELLS_PENCILS) || op.bundle_steps == 0)
-      return false;
-    if (op.bundle_width == 4) return launch_width(q, std::integral_constant<int, 4>());
-    return launch_width(q, std::integral_constant<int, 1>());
-  };
   // one workgroup per group of CPB cells, one launch per colour
   auto launch_simple = [&](auto kernel, auto q) {
     for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
@@ -146,7 +100,6 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
   };
   auto launch_any = [&](auto q) {
     using QOp = decltype(q);
-    if (launch_bundles(q)) return;
     if (op.cell_schedule == MFG_CELLS_GATHER) launch_gather(apply_cells_to_cells<P, CPB, Number, QOp>, q);
     else launch_simple(apply_cells<P, CPB, Number, QOp>, q);
   };
@@ -450,136 +403,6 @@ void OperatorData::colour_cells(const mfg_level_desc &d, int N3) {
   perm.assign(n_cells, 0);
   std::vector<int> fillp(colour_start.begin(), colour_start.end() - 1);
   for (int c = 0; c < n_cells; ++c) perm[fillp[colour[c]]++] = c;
-}
-
-// Structured-box detection and the bundle schedule (see apply_bundles).
-bool OperatorData::build_bundles(int width) {
-  bundle_steps = 0;
-  bundle_colour_start.clear();
-  const int N = p + 1, N3 = N * N * N;
-  if (width * N3 > 1024) return false;
-  const int32_t *cn = host_cell_nodes.data();
-  auto nd = [&](int c, int i, int j, int k) { return cn[(int64_t)c * N3 + i + N * (j + N * k)]; };
-  std::vector<int32_t> org((size_t)n_nodes, -1);
-  for (int c = 0; c < n_cells; ++c) {
-    const int32_t o = nd(c, 0, 0, 0);
-    if (org[o] >= 0) return false;
-    org[o] = c;
-  }
-  // neighbour across the +x / +y / +z face, checked on the whole face
-  auto neighbour = [&](int c, int dir) -> int {
-    const int32_t corner = dir == 0 ? nd(c, N - 1, 0, 0) : dir == 1 ? nd(c, 0, N - 1, 0) : nd(c, 0, 0, N - 1);
-    const int r = org[corner];
-    if (r < 0) return -1;
-    for (int a = 0; a < N; ++a)
-      for (int b = 0; b < N; ++b) {
-        const bool same = dir == 0   ? nd(c, N - 1, a, b) == nd(r, 0, a, b)
-                          : dir == 1 ? nd(c, a, N - 1, b) == nd(r, a, 0, b)
-                                     : nd(c, a, b, N - 1) == nd(r, a, b, 0);
-        if (!same) return -1;
-      }
-    return r;
-  };
-  std::vector<int32_t> nb[3], prev[3];
-  for (int dir = 0; dir < 3; ++dir) {
-    nb[dir].assign(n_cells, -1);
-    prev[dir].assign(n_cells, -1);
-  }
-  for (int c = 0; c < n_cells; ++c)
-    for (int dir = 0; dir < 3; ++dir) {
-      const int r = neighbour(c, dir);
-      nb[dir][c] = r;
-      if (r >= 0) {
-        if (prev[dir][r] >= 0) return false;
-        prev[dir][r] = c;
-      }
-    }
-  int origin = -1;
-  for (int c = 0; c < n_cells; ++c)
-    if (prev[0][c] < 0 && prev[1][c] < 0 && prev[2][c] < 0) {
-      if (origin >= 0) return false;
-      origin = c;
-    }
-  if (origin < 0) return false;
-  int ext[3];
-  for (int dir = 0; dir < 3; ++dir) {
-    ext[dir] = 1;
-    for (int c = origin; nb[dir][c] >= 0; c = nb[dir][c]) ++ext[dir];
-  }
-  const int nx = ext[0], ny = ext[1], nz = ext[2];
-  if ((int64_t)nx * ny * nz != n_cells) return false;
-  std::vector<int32_t> at((size_t)n_cells, -1);  // input cell at (i, j, k)
-  for (int k = 0, ck = origin; k < nz; ++k, ck = ck >= 0 ? nb[2][ck] : -1)
-    for (int j = 0, cj = ck; j < ny; ++j, cj = cj >= 0 ? nb[1][cj] : -1)
-      for (int i = 0, ci = cj; i < nx; ++i, ci = ci >= 0 ? nb[0][ci] : -1) {
-        if (ci < 0) return false;
-        at[i + (int64_t)nx * (j + (int64_t)ny * k)] = ci;
-      }
-  auto cell_at = [&](int i, int j, int k) { return at[i + (int64_t)nx * (j + (int64_t)ny * k)]; };
-  for (int k = 0; k < nz; ++k)
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) {
-        const int c = cell_at(i, j, k);
-        if (nb[0][c] != (i + 1 < nx ? cell_at(i + 1, j, k) : -1)) return false;
-        if (nb[1][c] != (j + 1 < ny ? cell_at(i, j + 1, k) : -1)) return false;
-        if (nb[2][c] != (k + 1 < nz ? cell_at(i, j, k + 1) : -1)) return false;
-      }
-  std::vector<int32_t> stored((size_t)n_cells);
-  for (int s = 0; s < n_cells; ++s) stored[perm[s]] = s;
-  // segments of S steps along x; colour = x-segment, y-bundle and z-bundle parity
-  const int S = std::min(8, nx), W = width == 4 ? 2 : 1;  // W: cells per bundle side
-  const int nsx = (nx + S - 1) / S, nby = (ny + W - 1) / W, nbz = (nz + W - 1) / W;
-  std::vector<int32_t> cells_tab, lens;
-  std::vector<uint32_t> bmap((size_t)nqp);  // stored-order map with constraint bits, first-touch bits below
-  for (int64_t sc = 0; sc < n_cells; ++sc)
-    for (int pt = 0; pt < N3; ++pt) {
-      const int32_t node = cn[(int64_t)perm[sc] * N3 + pt];
-      bmap[(size_t)sc * N3 + pt] = (uint32_t)node | ((uint32_t)host_cmask[node] << kConstraintShift);
-    }
-  std::vector<char> touched((size_t)n_nodes, 0);
-  bundle_colour_start.assign(1, 0);
-  for (int col = 0; col < 8; ++col) {
-    for (int bz = 0; bz < nbz; ++bz)
-      for (int by = 0; by < nby; ++by)
-        for (int sx = 0; sx < nsx; ++sx) {
-          if ((sx & 1) + 2 * (by & 1) + 4 * (bz & 1) != col) continue;
-          const int len = std::min(S, nx - sx * S);
-          lens.push_back(len);
-          for (int t = 0; t < S; ++t)
-            for (int lc = 0; lc < width; ++lc) {
-              const int j = W * by + (lc & 1), k = W * bz + (lc >> 1);
-              const bool present = t < len && j < ny && k < nz;
-              cells_tab.push_back(present ? stored[cell_at(sx * S + t, j, k)] : -1);
-            }
-          // the entry that stores a node first gets bit 31 (same rule as the kernel)
-          for (int t = 0; t < len; ++t)
-            for (int lc = 0; lc < width; ++lc) {
-              const int cy = lc & 1, cz = lc >> 1, j = W * by + cy, k = W * bz + cz;
-              if (j >= ny || k >= nz) continue;
-              const int64_t sc = stored[cell_at(sx * S + t, j, k)];
-              for (int pt = 0; pt < N3; ++pt) {
-                const int ii = pt % N, jj = (pt / N) % N, kk = pt / (N * N);
-                const bool owner = width == 1 || !((cy == 1 && jj == 0) || (cz == 1 && kk == 0));
-                const bool deferred = owner && ii == N - 1 && t + 1 < len;
-                if (!owner || deferred) continue;
-                uint32_t &e = bmap[(size_t)sc * N3 + pt];
-                const uint32_t node = e & kNodeMask;
-                if (!touched[node]) {
-                  e |= 1u << 31;
-                  touched[node] = 1;
-                }
-              }
-            }
-        }
-    bundle_colour_start.push_back((int)lens.size());
-  }
-  upload(bundle_cells, cells_tab.data(), cells_tab.size(), stream);
-  upload(bundle_len, lens.data(), lens.size(), stream);
-  upload(bundle_map, bmap.data(), bmap.size(), stream);
-  ++buffers_version;
-  bundle_steps = S;
-  bundle_width = width;
-  return true;
 }
 
 void OperatorData::build_gather() {
@@ -1015,14 +838,11 @@ int32_t mfg_operator_n_colours(mfg_operator op) { return op ? (int32_t)op->colou
 int mfg_operator_set_cell_schedule(mfg_operator op, int32_t schedule) {
   return guarded([&] {
     MFG_REQUIRE(op, MFG_ERR_ARGUMENT, "null operator");
-    MFG_REQUIRE(schedule == MFG_CELLS_COLOURS || schedule == MFG_CELLS_BUNDLES || schedule == MFG_CELLS_PENCILS ||
-                    schedule == MFG_CELLS_GATHER,
-                MFG_ERR_ARGUMENT, "unknown cell schedule");
+    MFG_REQUIRE(schedule == MFG_CELLS_COLOURS || schedule == MFG_CELLS_GATHER, MFG_ERR_ARGUMENT,
+                "unknown cell schedule");
     op->cell_schedule = MFG_CELLS_COLOURS;
     if (schedule == MFG_CELLS_GATHER) {
       if (!op->contrib.ptr) op->build_gather();
-      op->cell_schedule = schedule;
-    } else if (schedule != MFG_CELLS_COLOURS && op->build_bundles(schedule == MFG_CELLS_BUNDLES ? 4 : 1)) {
       op->cell_schedule = schedule;
     }
     ++op->buffers_version;  // another kernel path: captured graphs are stale

@@ -48,16 +48,10 @@ struct OperatorData {
   std::vector<int> colour_start;
   DeviceBuffer perm_dev;
   void colour_cells(const mfg_level_desc &d, int N3);
-  // bundle schedule of vmult on structured boxes (segments of 2x2-cell steps
-  // along x, coloured); empty when the mesh is not a structured box
-  int bundle_steps = 0, bundle_width = 4;  // steps per segment, cells per step (4 or 1)
-  std::vector<int> bundle_colour_start;
-  DeviceBuffer bundle_cells, bundle_len, bundle_map;
   int cell_schedule = MFG_CELLS_COLOURS;
   // bumped whenever a device buffer that apply() reads is reallocated (graphs
   // that captured the old addresses must be re-captured)
   uint64_t buffers_version = 0;
-  bool build_bundles(int width);  // false if the mesh is not a structured box
   // GATHER schedule: per node, the points (stored cell * N3 + point) that
   // touch it in colour order (CSR), its constrained components, and the
   // per-point contribution buffer

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libmfgpu.so")
 
 FP64, FP32 = 0, 1
 CACHING = {"scalar_referential": 0, "tensor2": 1, "tensor4": 2, "tensor4_ns": 3}
-CELLS_COLOURS, CELLS_BUNDLES, CELLS_PENCILS, CELLS_GATHER = 0, 1, 2, 3
+CELLS_COLOURS, CELLS_GATHER = 0, 3
 NUMBERING = {"lexicographic": 0, "cellwise": 1}
 T2_FOLDED, T2_GRADIENT = 0, 1  # tensor2 formulation-1 point storage
 
@@ -364,7 +364,7 @@ class _Operator:
         return out.view(-1, nc)
 
     def set_cell_schedule(self, schedule):
-        """CELLS_COLOURS or CELLS_BUNDLES (structured boxes; falls back to colours elsewhere)."""
+        """CELLS_COLOURS (one launch per colour) or CELLS_GATHER (two launches, bitwise the same result)."""
         check(lib().mfg_operator_set_cell_schedule(self.h, schedule))
 
     def cell_schedule(self):

@@ -317,13 +317,13 @@ def test_solid_other_cachings(cuda, caching, p):
 
 @pytest.mark.parametrize("p,n", [(1, [4, 3, 2]), (2, [5, 2, 3]), (3, [3, 3, 3]), (4, [4, 2, 2])])
 def test_cell_schedules(cuda, p, n):
-    """Bundle and pencil schedules (odd counts give partial bundles and short
-    segments) against the colour schedule and the oracle; vmult_add too."""
+    """The GATHER schedule against the colour schedule (bitwise) and the
+    oracle; vmult_add too; repeated applications are deterministic."""
     gm, om, go, oo, u = solid_pair([0, 0, 0], [1.0, 0.6, 0.4], n, p)
     x = O.fill_uniform(4, om.n_dofs)
     ref = oo.vmult(x)
     out = {}
-    for sched in (mfgpu.CELLS_COLOURS, mfgpu.CELLS_BUNDLES, mfgpu.CELLS_PENCILS):
+    for sched in (mfgpu.CELLS_COLOURS, mfgpu.CELLS_GATHER):
         go.set_cell_schedule(sched)
         assert go.cell_schedule() == sched
         y = torch.full((om.n_dofs,), float("nan"), dtype=torch.float64, device="cuda")
@@ -336,18 +336,9 @@ def test_cell_schedules(cuda, p, n):
         y2 = torch.empty_like(y)
         go.vmult(y2, dev(x))
         assert torch.equal(y, y2)  # deterministic
-    assert rel(out[mfgpu.CELLS_BUNDLES], out[mfgpu.CELLS_COLOURS].cpu().numpy()) < 1e-14
-    # heat, FP32
-    hm = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], n, p, 1, 63, 1)
-    hom = O.Mesh([0, 0, 0], [1, 1, 1], n, p, 1, 63, 1)
-    h = mfgpu.LaplaceOperator(hm, 0.1, None, precision=mfgpu.FP32)
-    xh = O.fill_uniform(5, hom.n_dofs)
-    href = O.Heat(hom, dt=0.1).vmult(xh)
-    for sched in (mfgpu.CELLS_BUNDLES, mfgpu.CELLS_PENCILS):
-        h.set_cell_schedule(sched)
-        yh = torch.empty(hom.n_dofs, dtype=torch.float32, device="cuda")
-        h.vmult(yh, dev(xh, torch.float32))
-        assert rel(yh.double(), href) < TOL32
+    assert torch.equal(out[mfgpu.CELLS_GATHER], out[mfgpu.CELLS_COLOURS])
+    with pytest.raises(mfgpu.MfgError):  # the removed segment schedules are refused
+        go.set_cell_schedule(1)
 
 
 @pytest.mark.parametrize("kind,p,precision,caching,form", [
@@ -445,7 +436,7 @@ def test_gradient_point_storage(cuda, p, precision, n):
         d = go.diagonal()
         assert rel(d.double().cpu().numpy()[dof], dref) < tol
         ys = {}
-        for sched in (mfgpu.CELLS_COLOURS, mfgpu.CELLS_GATHER, mfgpu.CELLS_BUNDLES):
+        for sched in (mfgpu.CELLS_COLOURS, mfgpu.CELLS_GATHER):
             go.set_cell_schedule(sched)
             y = torch.full((om.n_dofs,), float("nan"), dtype=go.dtype, device="cuda")
             go.vmult(y, dev(xg, go.dtype))
