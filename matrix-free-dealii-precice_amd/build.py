@@ -18,7 +18,12 @@ SOURCES = ["setup.cpp", "operator.hip", "vectors.hip", "multigrid.hip", "newton.
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from forming v_pk_* FP32 instructions.
-FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-munsafe-fp-atomics",
+# -ffp-contract=on: a multiply and an add fuse only within one source
+# expression, so that the rounding of a kernel's arithmetic follows from its
+# source alone and not from the code around it (the batched kernels and the
+# single-operator kernels, the colour and the GATHER schedule share their
+# arithmetic and stay bitwise equal).
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=on", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
 

@@ -145,20 +145,24 @@ __device__ inline void evaluate_vg(const Tile<N, Number> &T, const Number *S, co
 }
 
 // Integrate submitted values and reference gradients back to the nodes
-// (transpose of evaluate_vg).  Uses all 3NC slots.
-template <int N, int NC, typename Number>
+// (transpose of evaluate_vg).  Uses all 3NC slots; with kSpareSlots the tile
+// has 5NC slots and the gradients go to slots 2NC.. that evaluate_vg does not
+// use, which makes the two barriers that only guard slot reuse (after
+// evaluate_vg's last reads and after the gradient lines) unnecessary.
+template <int N, int NC, typename Number, bool kSpareSlots = false>
 __device__ inline void integrate_vg(const Tile<N, Number> &T, const Number *S, const Number *D,
                                     const Number (&val)[NC], const Number (&grad)[NC][3], Number (&out)[NC]) {
-  __syncthreads();  // B slots may still be read by evaluate_vg's last lines
+  constexpr int G0 = kSpareSlots ? 2 * NC : 0;  // first gradient slot
+  if constexpr (!kSpareSlots) __syncthreads();  // B slots may still be read by evaluate_vg's last lines
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int f = 0; f < 3; ++f) T.put(3 * c + f, grad[c][f]);
+    for (int f = 0; f < 3; ++f) T.put(G0 + 3 * c + f, grad[c][f]);
   __syncthreads();
   Number r[NC];
   auto gradient_t = [&](int c) {
-    r[c] = val[c] + T.template line_t<0>(D, T.i, 3 * c + 0) + T.template line_t<1>(D, T.j, 3 * c + 1) +
-           T.template line_t<2>(D, T.k, 3 * c + 2);
+    r[c] = val[c] + T.template line_t<0>(D, T.i, G0 + 3 * c + 0) + T.template line_t<1>(D, T.j, G0 + 3 * c + 1) +
+           T.template line_t<2>(D, T.k, G0 + 3 * c + 2);
   };
   if constexpr (rolled_components<N>()) {
 #pragma unroll 1
@@ -167,7 +171,7 @@ __device__ inline void integrate_vg(const Tile<N, Number> &T, const Number *S, c
 #pragma unroll
     for (int c = 0; c < NC; ++c) gradient_t(c);
   }
-  __syncthreads();
+  if constexpr (!kSpareSlots) __syncthreads();
 #pragma unroll
   for (int c = 0; c < NC; ++c) T.put(c, r[c]);
   __syncthreads();
@@ -307,7 +311,11 @@ struct QSolidT2F1 {
 //   F = I + Grad_xi u J_ref^{-1},   tau = mu F F^T - s I,
 // and JxW_ref = det J_ref w_q.  Fields: Grad_xi u [d][f] (9), s; stored in
 // pairs of fields (below).
-template <typename Number>
+// kEarlyCell (the cell index is the same on every lane: one cell per
+// workgroup): the cell's J_ref, J_ref^{-1} and det J_ref are loaded into
+// SGPRs with the point data, so that their latency overlaps the gather's
+// instead of following the evaluate sweeps.
+template <typename Number, bool kEarlyCell = false>
 struct QSolidT2F1G {
   static constexpr int NC = 3;
   static constexpr int kFields = 10;
@@ -321,6 +329,7 @@ struct QSolidT2F1G {
   struct Data {
     Number gu[9], s, w;
     const Number *jac;
+    Number jr[kEarlyCell ? 18 : 1];
   };
   // fields are stored in pairs, qd[((f / 2) nqp + q) 2 + f % 2]: one 16-byte
   // (FP64) load brings two fields of a point
@@ -339,17 +348,38 @@ struct QSolidT2F1G {
         d.s = x.y;
       }
     }
-    d.w = as_constant(cell_jxw)[cell] * wq[pt];
-    d.jac = cell_jac + (int64_t)cell * 18;
+    load_cell(d, cell, pt);
     return d;
   }
+  // the per-cell part of load(): J_ref, J_ref^{-1} and JxW_ref at point pt
+  __device__ void load_cell(Data &d, int cell, int pt) const {
+    d.jac = cell_jac + (int64_t)cell * 18;
+    if constexpr (kEarlyCell) {
+      Number jxw = as_constant(cell_jxw)[cell];
+#pragma unroll
+      for (int f = 0; f < 18; ++f) d.jr[f] = as_constant(d.jac)[f];
+      // all 19 values in SGPRs here: one wait, before the first barrier
+      asm volatile(""
+                   : "+s"(jxw), "+s"(d.jr[0]), "+s"(d.jr[1]), "+s"(d.jr[2]), "+s"(d.jr[3]), "+s"(d.jr[4]),
+                     "+s"(d.jr[5]), "+s"(d.jr[6]), "+s"(d.jr[7]), "+s"(d.jr[8]), "+s"(d.jr[9]), "+s"(d.jr[10]),
+                     "+s"(d.jr[11]), "+s"(d.jr[12]), "+s"(d.jr[13]), "+s"(d.jr[14]), "+s"(d.jr[15]), "+s"(d.jr[16]),
+                     "+s"(d.jr[17]));
+      d.w = jxw * wq[pt];
+    } else {
+      d.w = as_constant(cell_jxw)[cell] * wq[pt];
+    }
+  }
   __device__ void apply(const Data &d, Number (&val)[3], Number (&grad)[3][3]) const {
-    const ConstantPtr<Number> Jr = as_constant(d.jac), Jri = as_constant(d.jac) + 9;
+    // J_ref (f < 9) and J_ref^{-1} (9 + f) of the cell
+    auto jr = [&](int f) -> Number {
+      if constexpr (kEarlyCell) return d.jr[f];
+      else return as_constant(d.jac)[f];
+    };
     Number ji[9];
     {
       Number Jc[3][3];
 #pragma unroll
-      for (int f = 0; f < 9; ++f) Jc[f / 3][f % 3] = Jr[f] + d.gu[f];
+      for (int f = 0; f < 9; ++f) Jc[f / 3][f % 3] = jr(f) + d.gu[f];
       const Number id =
           Number(1) / (Jc[0][0] * (Jc[1][1] * Jc[2][2] - Jc[1][2] * Jc[2][1]) -
                        Jc[0][1] * (Jc[1][0] * Jc[2][2] - Jc[1][2] * Jc[2][0]) + Jc[0][2] * (Jc[1][0] * Jc[2][1] - Jc[1][1] * Jc[2][0]));
@@ -370,7 +400,7 @@ struct QSolidT2F1G {
       for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int e = 0; e < 3; ++e)
-          F[a][e] = d.gu[a * 3 + 0] * Jri[0 * 3 + e] + d.gu[a * 3 + 1] * Jri[1 * 3 + e] + d.gu[a * 3 + 2] * Jri[2 * 3 + e] +
+          F[a][e] = d.gu[a * 3 + 0] * jr(9 + 0 * 3 + e) + d.gu[a * 3 + 1] * jr(9 + 1 * 3 + e) + d.gu[a * 3 + 2] * jr(9 + 2 * 3 + e) +
                     (a == e ? Number(1) : Number(0));
 #pragma unroll
       for (int s6 = 0; s6 < 6; ++s6) {
@@ -781,7 +811,7 @@ __device__ inline void store_node(Number *p, const Number (&v)[NC]) {
   *reinterpret_cast<NodeValues<Number, NC> *>(p) = x;
 }
 
-template <int P, typename Number, class QOp, bool kToCells = false>
+template <int P, typename Number, class QOp, bool kToCells = false, bool kSpareSlots = false>
 __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ cell_nodes,
                                   const Number *__restrict__ src, Number *__restrict__ dst, int cell_begin,
                                   int cell_end, int cpb, int assign, const Number *s_shape, const Number *s_colloc,
@@ -793,7 +823,7 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   const int lc = pad ? cpb - 1 : lc_raw;
   const int pt = pad ? 0 : t - lc_raw * N3;
   Tile<N, Number> T;
-  T.base = s_tile + lc * 3 * NC * N3;
+  T.base = s_tile + lc * (kSpareSlots ? 5 : 3) * NC * N3;
   T.pt = pt;
   T.i = pt % N;
   T.j = (pt / N) % N;
@@ -823,7 +853,8 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   // two (measured); for the heat op and the 10-word tensor2 op, whose wave
   // count it does not change, and elsewhere the early read is.
   constexpr bool kLateDst = sizeof(Number) == 8 && P <= 4 && !std::is_same<QOp, QHeat<Number>>::value &&
-                            !std::is_same<QOp, QSolidT2F1G<Number>>::value;
+                            !std::is_same<QOp, QSolidT2F1G<Number>>::value &&
+                            !std::is_same<QOp, QSolidT2F1G<Number, true>>::value;
   const bool first = assign && (entry >> 31);
   Number dold[NC];
   if constexpr (!kLateDst && !kToCells) {
@@ -833,6 +864,11 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
 #pragma unroll
     for (int c = 0; c < NC; ++c) dold[c] = v.v[c];
   }
+  // the cached nodal field of the nodal-field ops, gathered with src
+  NodeValues<Number, 3> w{};
+  if constexpr (QOp::kNodalField) {
+    if (active) w = load_node<Number, 3>(qop.u + node * 3);
+  }
   // one cell per workgroup: the cell index is the same on every lane (padding
   // lanes included), which lets the per-cell tables come in as scalar loads
   const int qcell = cpb == 1 ? __builtin_amdgcn_readfirstlane(cell < cell_end ? cell : 0) : (active ? cell : 0);
@@ -841,10 +877,8 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   if constexpr (QOp::kNodalField) {
     // reference gradient of the cached nodal field (read plain: constrained
     // entries as stored), mf_nh_operator.h:873-877
-    Number w[3], wg[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) w[c] = active ? qop.u[node * 3 + c] : Number(0);
-    evaluate_g<N, 3, Number>(T, s_shape, s_colloc, w, wg);
+    Number wg[3][3];
+    evaluate_g<N, 3, Number>(T, s_shape, s_colloc, w.v, wg);
     qop.attach(qd, wg);
   }
 
@@ -852,7 +886,7 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   evaluate_vg<N, NC, Number>(T, s_shape, s_colloc, u, val, grad);
   qop.apply(qd, val, grad);
   Number out[NC];
-  integrate_vg<N, NC, Number>(T, s_shape, s_colloc, val, grad, out);
+  integrate_vg<N, NC, Number, kSpareSlots>(T, s_shape, s_colloc, val, grad, out);
   if constexpr (kToCells) {
     if (active) store_node<Number, NC>(dst + qidx * NC, out);
     return;
@@ -877,12 +911,20 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   }
 }
 
+// Spare tile slots (5 NC instead of 3 NC per cell, two barriers fewer per
+// cell) where the larger tile stays within 16 KB per workgroup.
+template <int P, int CPB, typename Number, int NC>
+constexpr bool spare_slots() {
+  return CPB * 5 * NC * (P + 1) * (P + 1) * (P + 1) * (int)sizeof(Number) <= 16384;
+}
+
 template <int P, int CPB, typename Number, class QOp>
 __global__ void __launch_bounds__((block_threads<P, CPB>()))
     apply_cells(ApplyArgs<Number> a, QOp qop, int assign) {
   constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC, NT = block_threads<P, CPB>();
+  constexpr bool kSpare = spare_slots<P, CPB, Number, NC>();
   __shared__ Number s_shape[N * N], s_colloc[N * N];
-  __shared__ Number s_tile[CPB * 3 * NC * N3];
+  __shared__ Number s_tile[CPB * (kSpare ? 5 : 3) * NC * N3];
   const int t = threadIdx.x;
   if (t < N * N) {
     s_shape[t] = a.shape[t];
@@ -898,7 +940,7 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
     }
   }
   const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
-  cell_group<P, Number, QOp>(qop, a.cell_nodes, a.src, a.dst, first_cell, a.cell_end, CPB, assign, s_shape,
+  cell_group<P, Number, QOp, false, kSpare>(qop, a.cell_nodes, a.src, a.dst, first_cell, a.cell_end, CPB, assign, s_shape,
                              s_colloc, s_tile);
 }
 
@@ -908,15 +950,16 @@ template <int P, int CPB, typename Number, class QOp>
 __global__ void __launch_bounds__((block_threads<P, CPB>()))
     apply_cells_to_cells(ApplyArgs<Number> a, QOp qop, Number *contrib) {
   constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC;
+  constexpr bool kSpare = spare_slots<P, CPB, Number, NC>();  // as apply_cells: the same arithmetic
   __shared__ Number s_shape[N * N], s_colloc[N * N];
-  __shared__ Number s_tile[CPB * 3 * NC * N3];
+  __shared__ Number s_tile[CPB * (kSpare ? 5 : 3) * NC * N3];
   const int t = threadIdx.x;
   if (t < N * N) {
     s_shape[t] = a.shape[t];
     s_colloc[t] = a.colloc[t];
   }
   const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
-  cell_group<P, Number, QOp, true>(qop, a.cell_nodes, a.src, contrib, first_cell, a.cell_end, CPB, 1, s_shape,
+  cell_group<P, Number, QOp, true, kSpare>(qop, a.cell_nodes, a.src, contrib, first_cell, a.cell_end, CPB, 1, s_shape,
                                    s_colloc, s_tile);
 }
 

@@ -33,9 +33,9 @@ const Number *t2_cell_jxw(const OperatorData &op) {
 }
 
 // tensor2 formulation 1 from the 10-word storage (affine meshes only)
-template <typename Number>
-QSolidT2F1G<Number> t2_gradient_op(const OperatorData &op) {
-  return QSolidT2F1G<Number>{op.qd.as<Number>(),         op.nqp,
+template <typename Number, bool kEarlyCell = false>
+QSolidT2F1G<Number, kEarlyCell> t2_gradient_op(const OperatorData &op) {
+  return QSolidT2F1G<Number, kEarlyCell>{op.qd.as<Number>(),         op.nqp,
                              Number(op.sp.mu),           Number(2.0 * op.lambda),
                              Number(op.rho_alpha),       op.cell_jac.as<Number>(),
                              op.cell_jxw.as<Number>(),   op.wq3.as<Number>()};
@@ -115,7 +115,7 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
       QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
       launch_any(q);
     } else if (op.t2_gradient()) {
-      launch_any(t2_gradient_op<Number>(op));
+      launch_any(t2_gradient_op<Number, CPB == 1>(op));
     } else if (op.sp.formulation == 1) {
       QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), ra, t2_cell_jxw<Number>(op),
                            op.wq3.as<Number>()};

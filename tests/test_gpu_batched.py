@@ -67,7 +67,10 @@ def test_batched_vmult_equals_per_instance(cuda, precision, storage, schedule):
     b.vmult_add(acc, srcs)
     b.synchronize()
     for i, (a, r) in enumerate(zip(acc, ref)):
-        assert torch.allclose(a, 2 * r, rtol=1e-6 if precision == mfgpu.FP32 else 1e-14, atol=0), i
+        # in the norm: entries of A src that are sums of cancelling cell
+        # contributions round differently from r + r
+        err = float((a.double() - 2 * r.double()).norm() / (2 * r.double()).norm())
+        assert err < (1e-6 if precision == mfgpu.FP32 else 1e-14), (i, err)
 
 
 def test_batched_cells_per_task_and_tuner(cuda):
