@@ -66,6 +66,11 @@ const char *mfg_version(void);
  *   cell_vertices:    n_cells*8*3 vertex coordinates, vertex v = vx+2vy+4vz;
  *                     geometry is the trilinear map (MappingQ1)
  *   constrained_dofs: sorted homogeneous Dirichlet dofs (identity rows)
+ *   material_ids:     per cell, or NULL for all 0 (cell->material_id(); the
+ *                     solid operator uses the inclusion material of
+ *                     mfg_solid_params on cells whose id is not 0,
+ *                     mf_nh_operator.h:404, 656-658; MatrixFree batches
+ *                     cells by it, mf_elasticity.h:837-855)
  * Node indices must be < 2^28.  n_q_1d must equal degree+1. */
 typedef struct mfg_level_desc {
   int32_t degree;
@@ -77,6 +82,7 @@ typedef struct mfg_level_desc {
   const double *cell_vertices;
   const int32_t *constrained_dofs;
   int64_t n_constrained;
+  const int32_t *material_ids;
 } mfg_level_desc;
 
 /* Structured box mesh builder (replaces GridGenerator::subdivided_hyper_rectangle
@@ -94,6 +100,11 @@ int mfg_box_mesh_create(const double lo[3], const double hi[3], const int32_t n_
 int mfg_box_mesh_create_bc(const double lo[3], const double hi[3], const int32_t n_cells[3], int32_t degree,
                            int32_t n_components, const int32_t face_comp_mask[6], mfg_mesh *out);
 int mfg_box_mesh_destroy(mfg_mesh mesh);
+/* Material id per cell (cell order cx + n0 (cy + n1 cz), n_cells entries;
+ * NULL resets all to 0), handed to operators through mfg_box_mesh_describe
+ * (cell->set_material_id of the reference's cases, turek_hron.h:88-90;
+ * id 0 is the first material, any other id the inclusion material). */
+int mfg_box_mesh_set_material_ids(mfg_mesh mesh, const int32_t *ids);
 /* Fills desc with pointers into the mesh's host arrays (valid while mesh lives). */
 int mfg_box_mesh_describe(mfg_mesh mesh, mfg_level_desc *desc);
 /* Node coordinates, n_nodes*3 doubles, host memory. */
@@ -125,11 +136,17 @@ typedef struct mfg_operator_t *mfg_operator;
 /* Neo-Hookean material (material.h:78-94) and Newmark factor alpha_1 =
  * 1/(beta dt^2) (mf_elasticity.h:240); formulation 0 or 1; caching one of
  * MFG_CACHING_*.  Replaces NeoHookOperator::initialize + set_material
- * (mf_nh_operator.h:115-128, 331-388). */
+ * (mf_nh_operator.h:115-128, 331-388).
+ * inclusion_mu, inclusion_nu, inclusion_rho: set_material's second material,
+ * used on the cells whose material id is not 0 (material_inclusion,
+ * mf_nh_operator.h:121-128, 529-539; the reference builds it with 100 mu,
+ * mf_elasticity.h:470-476), same alpha_1 and formulation.  inclusion_mu = 0:
+ * the inclusion cells use the first material. */
 typedef struct mfg_solid_params {
   double mu, nu, rho, alpha_1;
   int32_t formulation;
   int32_t caching;
+  double inclusion_mu, inclusion_nu, inclusion_rho;
 } mfg_solid_params;
 
 int mfg_solid_create(const mfg_level_desc *level, const mfg_solid_params *params, int32_t precision,

@@ -150,15 +150,34 @@ class NeoHookOperator : public OperatorBase<Number> {
   void initialize(const mfg_level_desc &level, const std::string &caching, const Material &mat) {
     this->clear();
     // the "MF caching" names of the reference's parameter file (mf_nh_operator.h:346-387)
-    int32_t c;
-    if (caching == "scalar_referential") c = MFG_CACHING_SCALAR_REFERENTIAL;
-    else if (caching == "tensor2") c = MFG_CACHING_TENSOR2;
-    else if (caching == "tensor4") c = MFG_CACHING_TENSOR4;
-    else if (caching == "tensor4_ns") c = MFG_CACHING_TENSOR4_NS;
+    if (caching == "scalar_referential") caching_ = MFG_CACHING_SCALAR_REFERENTIAL;
+    else if (caching == "tensor2") caching_ = MFG_CACHING_TENSOR2;
+    else if (caching == "tensor4") caching_ = MFG_CACHING_TENSOR4;
+    else if (caching == "tensor4_ns") caching_ = MFG_CACHING_TENSOR4_NS;
     else throw ExcMfg(MFG_ERR_ARGUMENT, "Unknown caching");
-    mfg_solid_params p{mat.mu, mat.nu, mat.rho, mat.alpha_1, (int32_t)mat.formulation, c};
-    check(mfg_solid_create(&level, &p, precision_of<Number>(), &this->op_));
+    level_ = level;
+    create(mat, nullptr);
   }
+  // set_material(material, material_inclusion) (mf_nh_operator.h:529-539):
+  // the inclusion material acts on the cells whose material id (level's
+  // material_ids) is not 0.  Rebuilds the operator; call cache() afterwards.
+  void set_material(const Material &mat, const Material &inclusion) { create(mat, &inclusion); }
+
+ private:
+  void create(const Material &mat, const Material *inclusion) {
+    this->clear();
+    mfg_solid_params p{mat.mu, mat.nu, mat.rho, mat.alpha_1, (int32_t)mat.formulation, caching_, 0, 0, 0};
+    if (inclusion) {
+      p.inclusion_mu = inclusion->mu;
+      p.inclusion_nu = inclusion->nu;
+      p.inclusion_rho = inclusion->rho;
+    }
+    check(mfg_solid_create(&level_, &p, precision_of<Number>(), &this->op_));
+  }
+  mfg_level_desc level_{};
+  int32_t caching_ = MFG_CACHING_TENSOR2;
+
+ public:
   template <typename UNumber>
   void cache(const DeviceVector<UNumber> &displacement) {
     check(mfg_solid_cache(this->op_, displacement.data(), precision_of<UNumber>()));

@@ -159,6 +159,7 @@ struct ApplyGroup {
       MFG_REQUIRE(op->p >= 1 && op->p <= kMaxDegree, MFG_ERR_ARGUMENT, "unsupported degree");
       MFG_REQUIRE(op->kind == 1 || op->sp.caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
                   "batched solid operators use the tensor2 caching");
+      MFG_REQUIRE(!op->has_inclusion(), MFG_ERR_ARGUMENT, "batched solid operators have one material");
     }
   }
 

@@ -34,21 +34,23 @@ static void residual_launch(const OperatorData &op, const double *u, const doubl
   a.colloc = op.tab_colloc.as<double>();
   a.points = op.tab_points.as<double>();
   a.weights = op.tab_weights.as<double>();
-  a.mu = op.sp.mu;
-  a.lambda = op.lambda;
-  a.kappa = op.kappa;
-  a.c_1 = op.c_1;
-  a.rho = op.sp.rho;
   for (int d = 0; d < 3; ++d) a.body[d] = body[d];
   a.formulation = op.sp.formulation;
   a.error = op.flag.as<int>();
-  for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
-    a.cell_begin = op.colour_start[col];
-    a.cell_end = op.colour_start[col + 1];
-    const int grid = blocks_for(a.cell_end - a.cell_begin, CPB);
-    if (grid) residual_cells<P, CPB, double><<<grid, block_threads<P, CPB>(), 0, op.stream>>>(a);
+  // one launch per colour and material (the cell's material,
+  // mf_elasticity.h:1781-1784)
+  op.for_each_range([&](int, int b, int e, int m) {
+    const OperatorData::MaterialConstants &M = op.mats[m];
+    a.mu = M.mu;
+    a.lambda = M.lambda;
+    a.kappa = M.kappa;
+    a.c_1 = M.c_1;
+    a.rho = M.rho;
+    a.cell_begin = b;
+    a.cell_end = e;
+    residual_cells<P, CPB, double><<<blocks_for(e - b, CPB), block_threads<P, CPB>(), 0, op.stream>>>(a);
     MFG_HIP(hipGetLastError());
-  }
+  });
 }
 
 // R = residual(u, acc) (+ the interface traction over n_faces stored face
@@ -222,6 +224,9 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
   MFG_REQUIRE(newmark.dt > 0 && newmark.beta > 0, MFG_ERR_ARGUMENT, "bad Newmark parameters");
   MFG_REQUIRE(material.mu > 0 && material.nu > -1 && material.nu < 0.5 && material.rho >= 0, MFG_ERR_ARGUMENT,
               "bad material");
+  MFG_REQUIRE(material.inclusion_mu == 0 || (material.inclusion_mu > 0 && material.inclusion_nu > -1 &&
+                                             material.inclusion_nu < 0.5 && material.inclusion_rho >= 0),
+              MFG_ERR_ARGUMENT, "bad inclusion material");
   MFG_REQUIRE(material.formulation == 0 || material.formulation == 1, MFG_ERR_ARGUMENT,
               "Unknown material formulation");
   MFG_REQUIRE(material.caching >= MFG_CACHING_SCALAR_REFERENTIAL && material.caching <= MFG_CACHING_TENSOR4_NS,

@@ -32,19 +32,48 @@ const Number *t2_cell_jxw(const OperatorData &op) {
   return op.affine_ref ? op.cell_jxw.as<Number>() : nullptr;
 }
 
-// tensor2 formulation 1 from the 10-word storage (affine meshes only)
+// tensor2 formulation 1 from the 10-word storage (affine meshes only), with
+// the constants of material m
 template <typename Number, bool kEarlyCell = false>
-QSolidT2F1G<Number, kEarlyCell> t2_gradient_op(const OperatorData &op) {
-  return QSolidT2F1G<Number, kEarlyCell>{op.qd.as<Number>(),         op.nqp,
-                             Number(op.sp.mu),           Number(2.0 * op.lambda),
-                             Number(op.rho_alpha),       op.cell_jac.as<Number>(),
-                             op.cell_jxw.as<Number>(),   op.wq3.as<Number>()};
+QSolidT2F1G<Number, kEarlyCell> t2_gradient_op(const OperatorData &op, int m = 0) {
+  const OperatorData::MaterialConstants &M = op.mats[m];
+  return QSolidT2F1G<Number, kEarlyCell>{op.qd.as<Number>(),        op.nqp,
+                                         Number(M.mu),              Number(2.0 * M.lambda),
+                                         Number(M.rho_alpha),       op.cell_jac.as<Number>(),
+                                         op.cell_jxw.as<Number>(),  op.wq3.as<Number>()};
 }
 
 // per-cell geometry of the heat op on affine meshes, or null (per-point fields)
 template <typename Number>
 const Number *heat_cell_geo(const OperatorData &op) {
   return op.affine_ref ? op.cell_geo.as<Number>() : nullptr;
+}
+
+// fn(q) with the quadrature op of this operator for the cells of material m
+// (the heat op has one).  kEarlyCell: one cell per workgroup, the 10-word op
+// takes its per-cell tables into SGPRs.
+template <typename Number, bool kEarlyCell, class F>
+static void with_qop(const OperatorData &op, int m, F fn) {
+  if (op.kind == 1) {
+    fn(QHeat<Number>{op.qd.as<Number>(), op.nqp, Number(op.dt), heat_cell_geo<Number>(op), op.wq3.as<Number>()});
+    return;
+  }
+  const OperatorData::MaterialConstants &M = op.mats[m];
+  const Number ra = Number(M.rho_alpha);
+  if (op.sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {
+    fn(QSolidSR<Number>{op.qd.as<Number>(), op.nqp, op.u_nodal.as<Number>(), Number(M.mu), Number(2.0 * M.lambda), ra});
+  } else if (op.sp.caching == MFG_CACHING_TENSOR4) {
+    fn(QSolidT4<Number>{op.qd.as<Number>(), op.nqp, ra});
+  } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
+    fn(QSolidT4NS<Number>{op.qd.as<Number>(), op.nqp, ra});
+  } else if (op.t2_gradient()) {
+    fn(t2_gradient_op<Number, kEarlyCell>(op, m));
+  } else if (op.sp.formulation == 1) {
+    fn(QSolidT2F1<Number>{op.qd.as<Number>(), op.nqp, Number(2.0 * M.lambda), ra, t2_cell_jxw<Number>(op),
+                          op.wq3.as<Number>()});
+  } else {
+    fn(QSolidT2F0<Number>{op.qd.as<Number>(), op.nqp, Number(M.kappa), Number(M.c_1), ra});
+  }
 }
 
 template <typename Number>
@@ -70,64 +99,46 @@ static void launch_apply(const OperatorData &op, const void *src, void *dst, int
   a.n_constrained = op.n_constrained;
   constexpr int NT = block_threads<P, CPB>();
   const int assign = fix_mode == 1 ? 1 : 0;  // vmult: first touch stores, no zero fill
-  // one workgroup per group of CPB cells, one launch per colour
-  auto launch_simple = [&](auto kernel, auto q) {
-    for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
-      a.cell_begin = op.colour_start[col];
-      a.cell_end = op.colour_start[col + 1];
-      a.fix_mode = col == 0 ? fix_mode : 0;
-      const int groups = blocks_for(a.cell_end - a.cell_begin, CPB);
-      if (groups == 0) continue;
-      kernel<<<groups, NT, 0, stream>>>(a, q, assign);
+  if (op.cell_schedule != MFG_CELLS_GATHER) {
+    // one workgroup per group of CPB cells, one launch per colour (and
+    // material); the first launch also writes the identity rows
+    bool fix_pending = true;
+    op.for_each_range([&](int, int b, int e, int m) {
+      with_qop<Number, CPB == 1>(op, m, [&](auto q) {
+        a.cell_begin = b;
+        a.cell_end = e;
+        a.fix_mode = fix_pending ? fix_mode : 0;
+        fix_pending = false;
+        apply_cells<P, CPB, Number, decltype(q)><<<blocks_for(e - b, CPB), NT, 0, stream>>>(a, q, assign);
+        MFG_HIP(hipGetLastError());
+      });
+    });
+    return;
+  }
+  // GATHER: all cells (of one material per launch) into the contribution
+  // buffer, then the ordered per-node sums
+  a.fix_mode = 0;
+  auto cells = [&](int b, int e, int m) {
+    with_qop<Number, CPB == 1>(op, m, [&](auto q) {
+      a.cell_begin = b;
+      a.cell_end = e;
+      apply_cells_to_cells<P, CPB, Number, decltype(q)><<<blocks_for(e - b, CPB), NT, 0, stream>>>(
+          a, q, op.contrib.as<Number>());
       MFG_HIP(hipGetLastError());
-    }
+    });
   };
-  // GATHER: all cells in one launch into the contribution buffer, then the
-  // ordered per-node sums
-  auto launch_gather = [&](auto kernel, auto q) {
-    using QOp = decltype(q);
-    constexpr int NC = QOp::NC;
-    a.cell_begin = 0;
-    a.cell_end = op.n_cells;
-    a.fix_mode = 0;
-    kernel<<<blocks_for(op.n_cells, CPB), NT, 0, stream>>>(a, q, op.contrib.as<Number>());
-    MFG_HIP(hipGetLastError());
-    const int64_t nn = op.n_nodes;
-    gather_contributions<Number, NC><<<(int)((nn + 255) / 256), 256, 0, stream>>>(
+  if (op.has_inclusion()) op.for_each_range([&](int, int b, int e, int m) { cells(b, e, m); });
+  else cells(0, op.n_cells, 0);
+  const int64_t nn = op.n_nodes;
+  auto sums = [&](auto nc_tag) {
+    constexpr int NCv = decltype(nc_tag)::value;
+    gather_contributions<Number, NCv><<<(int)((nn + 255) / 256), 256, 0, stream>>>(
         op.contrib.as<Number>(), op.gather_offsets.as<int32_t>(), op.gather_points.as<int32_t>(),
         op.gather_cmask.as<uint8_t>(), (const Number *)src, (Number *)dst, nn, fix_mode);
     MFG_HIP(hipGetLastError());
   };
-  auto launch_any = [&](auto q) {
-    using QOp = decltype(q);
-    if (op.cell_schedule == MFG_CELLS_GATHER) launch_gather(apply_cells_to_cells<P, CPB, Number, QOp>, q);
-    else launch_simple(apply_cells<P, CPB, Number, QOp>, q);
-  };
-  if (op.kind == 0) {
-    const Number ra = Number(op.rho_alpha);
-    if (op.sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {
-      QSolidSR<Number> q{op.qd.as<Number>(), op.nqp, op.u_nodal.as<Number>(), Number(op.sp.mu), Number(2.0 * op.lambda), ra};
-      launch_any(q);
-    } else if (op.sp.caching == MFG_CACHING_TENSOR4) {
-      QSolidT4<Number> q{op.qd.as<Number>(), op.nqp, ra};
-      launch_any(q);
-    } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
-      QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
-      launch_any(q);
-    } else if (op.t2_gradient()) {
-      launch_any(t2_gradient_op<Number, CPB == 1>(op));
-    } else if (op.sp.formulation == 1) {
-      QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), ra, t2_cell_jxw<Number>(op),
-                           op.wq3.as<Number>()};
-      launch_any(q);
-    } else {
-      QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), ra};
-      launch_any(q);
-    }
-  } else {
-    QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt), heat_cell_geo<Number>(op), op.wq3.as<Number>()};
-    launch_any(q);
-  }
+  if (op.nc == 3) sums(std::integral_constant<int, 3>());
+  else sums(std::integral_constant<int, 1>());
 }
 
 template <int P, typename Number>
@@ -135,7 +146,6 @@ static void launch_cache(OperatorData &op, const void *u, int u_fp32) {
   constexpr int CPB = DefaultCPB<P, Number>::value;
   CacheArgs<Number> a;
   a.cell_nodes = op.cell_nodes.as<uint32_t>();
-  a.n_cells = op.n_cells;
   a.u = u;
   a.u_fp32 = u_fp32;
   a.vertices = op.vertices.as<double>();
@@ -145,16 +155,21 @@ static void launch_cache(OperatorData &op, const void *u, int u_fp32) {
   a.weights = op.tab_weights.as<double>();
   a.qd = op.qd.as<Number>();
   a.nqp = op.nqp;
-  a.mu = Number(op.sp.mu);
-  a.lambda = Number(op.lambda);
   a.formulation = op.sp.formulation;
   a.caching = op.sp.caching;
   a.jxw_ref_field = op.affine_ref ? 0 : 1;
   a.t2_gradient = op.t2_gradient() ? 1 : 0;
   a.error = op.flag.as<int>();
-  const int grid = blocks_for(op.n_cells, CPB);
-  cache_solid<P, CPB, Number><<<grid, block_threads<P, CPB>(), 0, op.stream>>>(a);
-  MFG_HIP(hipGetLastError());
+  auto cells = [&](int b, int e, int m) {
+    a.cell_begin = b;
+    a.cell_end = e;
+    a.mu = Number(op.mats[m].mu);
+    a.lambda = Number(op.mats[m].lambda);
+    cache_solid<P, CPB, Number><<<blocks_for(e - b, CPB), block_threads<P, CPB>(), 0, op.stream>>>(a);
+    MFG_HIP(hipGetLastError());
+  };
+  if (op.has_inclusion()) op.for_each_range([&](int, int b, int e, int m) { cells(b, e, m); });
+  else cells(0, op.n_cells, 0);
 }
 
 template <int P, typename Number>
@@ -166,38 +181,14 @@ static void launch_diagonal(OperatorData &op) {
   a.colloc = op.tab_colloc.as<Number>();
   a.diag = op.diag.as<Number>();
   constexpr int NT = ((P + 1) * (P + 1) * (P + 1) + 63) / 64 * 64;
-  for (size_t col = 0; col + 1 < op.colour_start.size(); ++col) {
-    a.cell_begin = op.colour_start[col];
-    a.cell_end = op.colour_start[col + 1];
-    const int grid = a.cell_end - a.cell_begin;
-    if (grid == 0) continue;
-    if (op.kind == 0) {
-      const Number ra = Number(op.rho_alpha);
-      if (op.sp.caching == MFG_CACHING_SCALAR_REFERENTIAL) {
-        QSolidSR<Number> q{op.qd.as<Number>(), op.nqp, op.u_nodal.as<Number>(), Number(op.sp.mu), Number(2.0 * op.lambda), ra};
-        diagonal_cells<P, Number, QSolidSR<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      } else if (op.sp.caching == MFG_CACHING_TENSOR4) {
-        QSolidT4<Number> q{op.qd.as<Number>(), op.nqp, ra};
-        diagonal_cells<P, Number, QSolidT4<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
-        QSolidT4NS<Number> q{op.qd.as<Number>(), op.nqp, ra};
-        diagonal_cells<P, Number, QSolidT4NS<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      } else if (op.t2_gradient()) {
-        diagonal_cells<P, Number, QSolidT2F1G<Number>><<<grid, NT, 0, op.stream>>>(a, t2_gradient_op<Number>(op));
-      } else if (op.sp.formulation == 1) {
-        QSolidT2F1<Number> q{op.qd.as<Number>(), op.nqp, Number(2.0 * op.lambda), Number(op.rho_alpha),
-                             t2_cell_jxw<Number>(op), op.wq3.as<Number>()};
-        diagonal_cells<P, Number, QSolidT2F1<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      } else {
-        QSolidT2F0<Number> q{op.qd.as<Number>(), op.nqp, Number(op.kappa), Number(op.c_1), Number(op.rho_alpha)};
-        diagonal_cells<P, Number, QSolidT2F0<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-      }
-    } else {
-      QHeat<Number> q{op.qd.as<Number>(), op.nqp, Number(op.dt), heat_cell_geo<Number>(op), op.wq3.as<Number>()};
-      diagonal_cells<P, Number, QHeat<Number>><<<grid, NT, 0, op.stream>>>(a, q);
-    }
-    MFG_HIP(hipGetLastError());
-  }
+  op.for_each_range([&](int, int b, int e, int m) {
+    with_qop<Number, false>(op, m, [&](auto q) {
+      a.cell_begin = b;
+      a.cell_end = e;
+      diagonal_cells<P, Number, decltype(q)><<<e - b, NT, 0, op.stream>>>(a, q);
+      MFG_HIP(hipGetLastError());
+    });
+  });
 }
 
 #define MFG_DISPATCH_DEGREE(p, FN, ...)                                                  \
@@ -370,7 +361,9 @@ static void upload_tables(OperatorData &op, const Tables1D &t) {
 // Greedy colouring of the cells in their given order such that no two cells
 // of one colour share a node; on a lexicographic hex grid this yields the
 // 8 parity colours.  Fills perm (colour-major order -> given cell index) and
-// colour_start.
+// colour_start; with an inclusion material (solid operators with
+// inclusion_mu != 0 and a cell whose material id is not 0) also mat_split,
+// the material-0 cells of each colour coming first.
 void OperatorData::colour_cells(const mfg_level_desc &d, int N3) {
   std::vector<int64_t> off((size_t)n_nodes + 1, 0);
   const int64_t ne = (int64_t)n_cells * N3;
@@ -401,8 +394,19 @@ void OperatorData::colour_cells(const mfg_level_desc &d, int N3) {
   for (int c = 0; c < n_cells; ++c) colour_start[colour[c] + 1]++;
   for (int k = 0; k < n_col; ++k) colour_start[k + 1] += colour_start[k];
   perm.assign(n_cells, 0);
+  mat_split.clear();
+  const int32_t *ids = kind == 0 && sp.inclusion_mu != 0 ? d.material_ids : nullptr;
+  const bool split = ids && std::any_of(ids, ids + n_cells, [](int32_t x) { return x != 0; });
   std::vector<int> fillp(colour_start.begin(), colour_start.end() - 1);
-  for (int c = 0; c < n_cells; ++c) perm[fillp[colour[c]]++] = c;
+  if (!split) {
+    for (int c = 0; c < n_cells; ++c) perm[fillp[colour[c]]++] = c;
+    return;
+  }
+  mat_split.assign(colour_start.begin(), colour_start.end() - 1);
+  for (int c = 0; c < n_cells; ++c)
+    if (ids[c] == 0) ++mat_split[colour[c]];
+  std::vector<int> fill1(mat_split);
+  for (int c = 0; c < n_cells; ++c) perm[ids[c] == 0 ? fillp[colour[c]]++ : fill1[colour[c]]++] = c;
 }
 
 void OperatorData::build_gather() {
@@ -627,11 +631,22 @@ void OperatorData::setup_heat(const mfg_level_desc &d, const double *coef) {
 
 void OperatorData::setup_solid() {
   // material.h:78-94
-  const double mu = sp.mu, nu = sp.nu;
-  kappa = (2.0 * mu * (1.0 + nu)) / (3.0 * (1.0 - 2.0 * nu));
-  c_1 = mu / 2.0;
-  lambda = (2.0 * mu * nu) / (1.0 - 2.0 * nu);
-  rho_alpha = sp.rho * sp.alpha_1;
+  auto constants = [&](double mu, double nu, double rho) {
+    MaterialConstants M;
+    M.mu = mu;
+    M.kappa = (2.0 * mu * (1.0 + nu)) / (3.0 * (1.0 - 2.0 * nu));
+    M.c_1 = mu / 2.0;
+    M.lambda = (2.0 * mu * nu) / (1.0 - 2.0 * nu);
+    M.rho = rho;
+    M.rho_alpha = rho * sp.alpha_1;
+    return M;
+  };
+  mats[0] = constants(sp.mu, sp.nu, sp.rho);
+  mats[1] = sp.inclusion_mu != 0 ? constants(sp.inclusion_mu, sp.inclusion_nu, sp.inclusion_rho) : mats[0];
+  kappa = mats[0].kappa;
+  c_1 = mats[0].c_1;
+  lambda = mats[0].lambda;
+  rho_alpha = mats[0].rho_alpha;
   switch (sp.caching) {
     case MFG_CACHING_SCALAR_REFERENTIAL: n_fields = QSolidSR<double>::kFields; break;
     case MFG_CACHING_TENSOR4: n_fields = QSolidT4<double>::kFields; break;
@@ -669,6 +684,9 @@ int mfg_solid_create(const mfg_level_desc *level, const mfg_solid_params *params
     MFG_REQUIRE(params->formulation == 1 || params->caching == MFG_CACHING_TENSOR2, MFG_ERR_ARGUMENT,
                 "formulation 0 needs the tensor2 caching");
     MFG_REQUIRE(params->mu > 0 && params->nu > -1 && params->nu < 0.5, MFG_ERR_ARGUMENT, "bad material");
+    MFG_REQUIRE(params->inclusion_mu == 0 ||
+                    (params->inclusion_mu > 0 && params->inclusion_nu > -1 && params->inclusion_nu < 0.5),
+                MFG_ERR_ARGUMENT, "bad inclusion material");
     auto op = new mfg_operator_t();
     try {
       op->kind = 0;

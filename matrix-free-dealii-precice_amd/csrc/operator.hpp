@@ -20,7 +20,14 @@ struct OperatorData {
   int64_t n_nodes = 0, n_dofs = 0, nqp = 0, n_constrained = 0;
   hipStream_t stream = nullptr;
   mfg_solid_params sp{};
-  double kappa = 0, c_1 = 0, lambda = 0, rho_alpha = 0, dt = 0;
+  double kappa = 0, c_1 = 0, lambda = 0, rho_alpha = 0, dt = 0;  // material 0
+  // neo-Hookean constants (material.h:78-94) of material 0 and of the
+  // inclusion material, used on cells whose material id is not 0
+  // (mf_nh_operator.h:404, 656-658)
+  struct MaterialConstants {
+    double mu = 0, lambda = 0, kappa = 0, c_1 = 0, rho = 0, rho_alpha = 0;
+  };
+  MaterialConstants mats[2];
   bool ready = false, has_diagonal = false;
 
   DeviceBuffer cell_nodes;   // uint32: node | constraint bits << 28
@@ -46,6 +53,22 @@ struct OperatorData {
   // colour k occupies stored cells [colour_start[k], colour_start[k+1])
   std::vector<int32_t> perm;
   std::vector<int> colour_start;
+  // with an inclusion material the cells of colour k are sorted by material
+  // (as MatrixFree batches cells by cell_vectorization_category,
+  // mf_elasticity.h:837-855): material 0 in [colour_start[k], mat_split[k]),
+  // the inclusion in [mat_split[k], colour_start[k+1]).  Empty otherwise.
+  std::vector<int> mat_split;
+  bool has_inclusion() const { return !mat_split.empty(); }
+  // f(colour, first stored cell, end, material) for every non-empty range of
+  // one colour and one material, in launch (stored) order
+  template <class F>
+  void for_each_range(F f) const {
+    for (size_t k = 0; k + 1 < colour_start.size(); ++k) {
+      const int b = colour_start[k], e = colour_start[k + 1], s = has_inclusion() ? mat_split[k] : e;
+      if (s > b) f((int)k, b, s, 0);
+      if (e > s) f((int)k, s, e, 1);
+    }
+  }
   DeviceBuffer perm_dev;
   void colour_cells(const mfg_level_desc &d, int N3);
   int cell_schedule = MFG_CELLS_COLOURS;

@@ -141,6 +141,7 @@ struct mfg_mesh_t {
   std::vector<int32_t> constrained;
   std::vector<double> node_x;
   std::vector<int32_t> lex_id;  // node -> its lexicographic number (identity until renumbered)
+  std::vector<int32_t> material_ids;  // per cell; empty: all 0
   int numbering = MFG_NUMBERING_LEXICOGRAPHIC;
 };
 
@@ -238,9 +239,19 @@ int mfg_box_mesh_destroy(mfg_mesh mesh) {
   return MFG_OK;
 }
 
+int mfg_box_mesh_set_material_ids(mfg_mesh m, const int32_t *ids) {
+  return guarded([&] {
+    MFG_REQUIRE(m, MFG_ERR_ARGUMENT, "null mesh");
+    const size_t n_cells = m->vertices.size() / 24;
+    if (ids) m->material_ids.assign(ids, ids + n_cells);
+    else m->material_ids.clear();
+  });
+}
+
 int mfg_box_mesh_describe(mfg_mesh m, mfg_level_desc *d) {
   return guarded([&] {
     MFG_REQUIRE(m && d, MFG_ERR_ARGUMENT, "null argument");
+    d->material_ids = m->material_ids.empty() ? nullptr : m->material_ids.data();
     d->degree = m->p;
     d->n_q_1d = m->p + 1;
     d->n_components = m->nc;

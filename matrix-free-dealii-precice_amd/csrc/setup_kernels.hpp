@@ -10,10 +10,10 @@ namespace mfg {
 template <typename Number>
 struct CacheArgs {
   const uint32_t *cell_nodes;
-  int n_cells;
+  int cell_begin, cell_end;  // stored cells of one material
   const void *u;     // displacement, precision u_fp32 ? float : double
   int u_fp32;
-  const double *vertices;  // n_cells * 24
+  const double *vertices;  // 24 per stored cell
   const Number *shape, *colloc;
   const double *points, *weights;  // Gauss rule on [0,1], N each
   Number *qd;
@@ -88,8 +88,8 @@ __global__ void __launch_bounds__((block_threads<P, CPB>())) cache_solid(CacheAr
   T.j = (pt / N) % N;
   T.k = pt / (N * N);
   T.writer = !pad;
-  const int cell = blockIdx.x * CPB + lc;
-  const bool active = !pad && cell < a.n_cells;
+  const int cell = a.cell_begin + blockIdx.x * CPB + lc;
+  const bool active = !pad && cell < a.cell_end;
   const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
   const int64_t node = active ? (int64_t)(a.cell_nodes[qidx] & kNodeMaskDev) : 0;
   Number u[3];

@@ -34,12 +34,21 @@ class MfgError(RuntimeError):
 class LevelDesc(ctypes.Structure):
     _fields_ = [("degree", ctypes.c_int32), ("n_q_1d", ctypes.c_int32), ("n_components", ctypes.c_int32),
                 ("n_cells", ctypes.c_int32), ("n_nodes", ctypes.c_int64), ("cell_nodes", _c_i32p),
-                ("cell_vertices", _c_dp), ("constrained_dofs", _c_i32p), ("n_constrained", ctypes.c_int64)]
+                ("cell_vertices", _c_dp), ("constrained_dofs", _c_i32p), ("n_constrained", ctypes.c_int64),
+                ("material_ids", _c_i32p)]
 
 
 class SolidParams(ctypes.Structure):
     _fields_ = [("mu", ctypes.c_double), ("nu", ctypes.c_double), ("rho", ctypes.c_double),
-                ("alpha_1", ctypes.c_double), ("formulation", ctypes.c_int32), ("caching", ctypes.c_int32)]
+                ("alpha_1", ctypes.c_double), ("formulation", ctypes.c_int32), ("caching", ctypes.c_int32),
+                ("inclusion_mu", ctypes.c_double), ("inclusion_nu", ctypes.c_double),
+                ("inclusion_rho", ctypes.c_double)]
+
+
+def _solid_params(mu, nu, rho, alpha_1, formulation, caching, inclusion=None):
+    """SolidParams; inclusion: (mu, nu, rho) of the material on cells whose material id is not 0."""
+    imu, inu, irho = inclusion if inclusion is not None else (0.0, 0.0, 0.0)
+    return SolidParams(mu, nu, rho, alpha_1, formulation, caching, imu, inu, irho)
 
 
 _LIB = None
@@ -55,6 +64,7 @@ SIGNATURES = {
     "mfg_box_mesh_create": (ctypes.c_int, [_c_dp, _c_dp, _c_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, ctypes.POINTER(_vp)]),
     "mfg_box_mesh_destroy": (ctypes.c_int, [_vp]),
+    "mfg_box_mesh_set_material_ids": (ctypes.c_int, [_vp, _c_i32p]),
     "mfg_box_mesh_describe": (ctypes.c_int, [_vp, ctypes.POINTER(LevelDesc)]),
     "mfg_box_mesh_node_coords": (ctypes.c_int, [_vp, _c_dp]),
     "mfg_box_mesh_renumber": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -181,6 +191,16 @@ def _check_pair(dst, src, n, dtype):
         raise ValueError("dst and src must be different vectors")
 
 
+def refined_material_ids(ids, n_cells, times):
+    """Material ids of the box mesh refined `times` times from one with n_cells = (n0, n1, n2) cells and
+    ids per cell: children inherit their parent's id, as deal.II's refinement does."""
+    import numpy as np
+    a = np.asarray(ids, dtype=np.int32).reshape(n_cells[2], n_cells[1], n_cells[0])
+    for _ in range(times):
+        a = a.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
+    return np.ascontiguousarray(a.reshape(-1))
+
+
 class BoxMesh:
     """Structured box mesh level (cells, nodes, Dirichlet dofs) built on the host."""
 
@@ -218,6 +238,20 @@ class BoxMesh:
         if getattr(self, "h", None):
             self._destroy(self.h)
             self.h = None
+
+    def set_material_ids(self, ids):
+        """Material id per cell (cell cx + n0 (cy + n1 cz)); None resets all to 0.  Operators built from the
+        mesh afterwards use the inclusion material on cells whose id is not 0."""
+        import numpy as np
+        if ids is None:
+            self._material_ids = None
+            check(lib().mfg_box_mesh_set_material_ids(self.h, None))
+        else:
+            self._material_ids = np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
+            if len(self._material_ids) != self.n_cells:
+                raise ValueError("one material id per cell")
+            check(lib().mfg_box_mesh_set_material_ids(self.h, self._material_ids.ctypes.data_as(_c_i32p)))
+        check(lib().mfg_box_mesh_describe(self.h, ctypes.byref(self.desc)))
 
     def cell_nodes(self):
         import numpy as np
@@ -395,8 +429,10 @@ class _Operator:
 class NeoHookOperator(_Operator):
     """Device counterpart of NeoHookOperator<3, Number> (mf_nh_operator.h:99-237)."""
 
-    def __init__(self, mesh, mu, nu, rho, alpha_1, formulation=1, caching="tensor2", precision=FP64):
-        prm = SolidParams(mu, nu, rho, alpha_1, formulation, CACHING[caching])
+    def __init__(self, mesh, mu, nu, rho, alpha_1, formulation=1, caching="tensor2", precision=FP64, inclusion=None):
+        """inclusion: (mu, nu, rho) of set_material's second material, on the mesh's cells whose material id
+        is not 0 (mf_nh_operator.h:121-128)."""
+        prm = _solid_params(mu, nu, rho, alpha_1, formulation, CACHING[caching], inclusion)
         h = _vp()
         check(lib().mfg_solid_create(ctypes.byref(mesh.desc), ctypes.byref(prm), precision, ctypes.byref(h)))
         super().__init__(h, precision, mesh)
@@ -679,12 +715,15 @@ class SolidProblem:
     def __init__(self, meshes, mu, nu, rho, beta=0.25, gamma=0.5, dt=1e-3, body=(0.0, 0.0, 0.0), formulation=1,
                  level_precision=FP64, max_newton=10, min_newton=3, tol_f=1e-8, tol_u=1e-6, tol_f_abs=0.0,
                  tol_u_abs=0.0, tol_lin=1e-6, max_lin=1000, cond_cg_iterations=20, cheb_degree=5, smoothing_range=20.0,
-                 eig_cg_iterations=30, coarse_kind=0, coarse_range=1e-3, coarse_tolerance=1e-3, caching="tensor2"):
+                 eig_cg_iterations=30, coarse_kind=0, coarse_range=1e-3, coarse_tolerance=1e-3, caching="tensor2",
+                 inclusion=None):
+        """inclusion: (mu, nu, rho) of the material on cells whose material id is not 0 (every level's mesh
+        carries its own ids: set_material_ids, refined_material_ids)."""
         L = len(meshes)
         descs = (LevelDesc * L)(*[m.desc for m in meshes])
         self._children = [box_children(meshes[l - 1], meshes[l]) for l in range(1, L)]
         ch = (_c_i32p * max(L - 1, 1))(*([c.ctypes.data_as(_c_i32p) for c in self._children] or [None]))
-        sp = SolidParams(mu, nu, rho, 0.0, formulation, CACHING[caching])
+        sp = _solid_params(mu, nu, rho, 0.0, formulation, CACHING[caching], inclusion)
         nm = NewmarkParams(beta, gamma, dt, (ctypes.c_double * 3)(*body), max_newton, min_newton, tol_f, tol_u,
                            tol_f_abs, tol_u_abs, tol_lin, max_lin, cond_cg_iterations)
         mp = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, coarse_range, coarse_tolerance)

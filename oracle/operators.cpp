@@ -124,13 +124,22 @@ struct Geometry {
 };
 
 // ------------------------------------------------------------- solid op ---
+// neo-Hookean constants of one material (material.h:78-94)
+struct Material {
+  double mu = 0, nu = 0, rho = 0, kappa = 0, c_1 = 0, lambda = 0, rho_alpha = 0;
+};
+
 struct Solid {
   Mesh *m = nullptr;
   Basis B;
   Geometry G;
   int Q3 = 0, form = 1, caching = 1;
-  double mu = 0, nu = 0, rho = 0, alpha1 = 0;
-  double kappa = 0, c_1 = 0, lambda = 0, rho_alpha = 0;
+  double alpha1 = 0;
+  // mat[0] on cells of material id 0, mat[1] (the inclusion) on the others
+  // (mf_nh_operator.h:404, 658); mat_id empty: every cell has id 0
+  Material mat[2];
+  vector<int32_t> mat_id;
+  const Material &material(int cell) const { return !mat_id.empty() && mat_id[cell] != 0 ? mat[1] : mat[0]; }
   // caches, per (cell, q)
   vector<double> sc, sc2, t2, t4, jinv_cur, jxw_cur;
   vector<double> u_cell;  // scalar_referential: plain cell dof values
@@ -147,12 +156,17 @@ struct Heat {
   vector<double> coef;  // per (cell, q)
 };
 
-static void material_constants(Solid &s) {
+static Material material_constants(double mu, double nu, double rho, double alpha1) {
   // material.h:78-94
-  s.kappa = (2.0 * s.mu * (1.0 + s.nu)) / (3.0 * (1.0 - 2.0 * s.nu));
-  s.c_1 = s.mu / 2.0;
-  s.rho_alpha = s.rho * s.alpha1;
-  s.lambda = (2.0 * s.mu * s.nu) / (1.0 - 2.0 * s.nu);
+  Material M;
+  M.mu = mu;
+  M.nu = nu;
+  M.rho = rho;
+  M.kappa = (2.0 * mu * (1.0 + nu)) / (3.0 * (1.0 - 2.0 * nu));
+  M.c_1 = mu / 2.0;
+  M.rho_alpha = rho * alpha1;
+  M.lambda = (2.0 * mu * nu) / (1.0 - 2.0 * nu);
+  return M;
 }
 
 static void gather(const Mesh &m, int cell, const double *src, bool zero_constrained,
@@ -191,7 +205,7 @@ static void deformation_gradient(const double *gxi /*[c][f] stride Q3*/, int Q3,
 }
 
 // Kirchhoff stress tau (material.h:116-161)
-static void get_tau(const Solid &s, const double F[3][3], double tau[3][3]) {
+static void get_tau(const Solid &s, const Material &M, const double F[3][3], double tau[3][3]) {
   const double J = det3(F);
   double b[3][3];
   for (int i = 0; i < 3; ++i)
@@ -201,17 +215,17 @@ static void get_tau(const Solid &s, const double F[3][3], double tau[3][3]) {
       b[i][j] = t;
     }
   if (s.form == 1) {
-    const double tmp = s.mu - 2.0 * s.lambda * std::log(J);
+    const double tmp = M.mu - 2.0 * M.lambda * std::log(J);
     for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) tau[i][j] = s.mu * b[i][j] - (i == j ? tmp : 0.0);
+      for (int j = 0; j < 3; ++j) tau[i][j] = M.mu * b[i][j] - (i == j ? tmp : 0.0);
   } else {
     const double Jm23 = std::pow(J, -2.0 / 3.0);
-    const double dpsi = (s.kappa / 2.0) * (J - 1.0 / J);
+    const double dpsi = (M.kappa / 2.0) * (J - 1.0 / J);
     double tr = 0;
-    for (int i = 0; i < 3; ++i) tr += 2.0 * s.c_1 * Jm23 * b[i][i];
+    for (int i = 0; i < 3; ++i) tr += 2.0 * M.c_1 * Jm23 * b[i][i];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
-        tau[i][j] = 2.0 * s.c_1 * Jm23 * b[i][j] + (i == j ? dpsi * J - tr / 3.0 : 0.0);
+        tau[i][j] = 2.0 * M.c_1 * Jm23 * b[i][j] + (i == j ? dpsi * J - tr / 3.0 : 0.0);
   }
 }
 
@@ -232,6 +246,11 @@ static int solid_cache(Solid &s, const double *u) {
   int bad = 0;
 #pragma omp parallel for schedule(static) reduction(+ : bad)
   for (int cell = 0; cell < m.n_cells; ++cell) {
+#ifdef ORC_PLANT_INCL_CACHE_MAT0
+    const Material &M = s.mat[0];  // planted: the cache ignores the cell's material
+#else
+    const Material &M = s.material(cell);
+#endif
     vector<double> loc, val, gr, xloc, xval, xgr;
     gather(m, cell, u, false, loc);  // read_dof_values_plain
     eval_vector(s.B, 3, N3, loc.data(), val, gr);
@@ -268,9 +287,9 @@ static int solid_cache(Solid &s, const double *u) {
         for (int i = 0; i < 9; ++i) s.t2[o * 9 + i] = F[i / 3][i % 3];
         continue;
       }
-      const double scalar = s.mu - 2.0 * s.lambda * std::log(J);
+      const double scalar = M.mu - 2.0 * M.lambda * std::log(J);
       double tau[3][3];
-      get_tau(s, F, tau);
+      get_tau(s, M, F, tau);
       switch (s.caching) {
         case 0:
           s.sc[o] = scalar;
@@ -291,7 +310,7 @@ static int solid_cache(Solid &s, const double *u) {
                   double S = 0.5 * ((i == k && j == l) + (i == l && j == k));
                   double IxI = (i == j && k == l) ? 1.0 : 0.0;
                   s.t4[o * 81 + ((i * 3 + j) * 3 + k) * 3 + l] =
-                      (scalar * 2.0 / J) * S + (s.lambda * 2.0 / J) * IxI;
+                      (scalar * 2.0 / J) * S + (M.lambda * 2.0 / J) * IxI;
                 }
           break;
         case 3: {  // mf_nh_operator.h:495-516
@@ -306,9 +325,9 @@ static int solid_cache(Solid &s, const double *u) {
                   // ikjl of (I, I): d_ik d_jl
                   double FiT_ij = Fi[j][i], FiT_kl = Fi[l][k], FiT_il = Fi[l][i], Fi_jk = Fi[j][k];
                   s.t4[o * 81 + ((i * 3 + j) * 3 + k) * 3 + l] =
-                      (2.0 * s.lambda) * FiT_ij * FiT_kl +
-                      (s.mu - 2.0 * s.lambda * lnJ) * FiT_il * Fi_jk +
-                      s.mu * ((i == k && j == l) ? 1.0 : 0.0);
+                      (2.0 * M.lambda) * FiT_ij * FiT_kl +
+                      (M.mu - 2.0 * M.lambda * lnJ) * FiT_il * Fi_jk +
+                      M.mu * ((i == k && j == l) ? 1.0 : 0.0);
                 }
           break;
         }
@@ -322,6 +341,7 @@ static int solid_cache(Solid &s, const double *u) {
 // mf_nh_operator.h:651-1019.
 static void solid_cell(const Solid &s, int cell, const double *loc, double *out) {
   const Mesh &m = *s.m;
+  const Material &M = s.material(cell);
   const int N3 = m.N * m.N * m.N, Q3 = s.Q3;
   vector<double> val, gr;
   eval_vector(s.B, 3, N3, loc, val, gr);
@@ -364,12 +384,12 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
         for (int j = 0; j < 3; ++j) sym[i][j] = 0.5 * (g[i][j] + g[j][i]);
       double tau_bar[3][3], tr_tau_bar = 0;
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) tau_bar[i][j] = bb[i][j] * (2.0 * s.c_1);
+        for (int j = 0; j < 3; ++j) tau_bar[i][j] = bb[i][j] * (2.0 * M.c_1);
       for (int i = 0; i < 3; ++i) tr_tau_bar += tau_bar[i][i];
       const double tr_tau_bar_dim = tr_tau_bar / 3.0;
-      const double dPsi = (s.kappa / 2.0) * (det_F - 1.0 / det_F);
+      const double dPsi = (M.kappa / 2.0) * (det_F - 1.0 / det_F);
       const double dPsi_J = dPsi * det_F;
-      const double d2Psi = (s.kappa / 2.0) * (1.0 + 1.0 / (det_F * det_F));
+      const double d2Psi = (M.kappa / 2.0) * (1.0 + 1.0 / (det_F * det_F));
       double tau[3][3];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) tau[i][j] = tau_bar[i][j] + (i == j ? dPsi_J - tr_tau_bar_dim : 0.0);
@@ -401,21 +421,21 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
           for (int k = 0; k < 3; ++k) geo += g[i][k] * tau[k][j];
           P[i][j] = (jc[i][j] + geo) * inv_det_F;
         }
-      for (int d = 0; d < 3; ++d) vs[d] = v[d] * s.rho_alpha * inv_det_F;
+      for (int d = 0; d < 3; ++d) vs[d] = v[d] * M.rho_alpha * inv_det_F;
     } else if (s.caching == 1) {
       // mf_nh_operator.h:945-974
       double sym[3][3];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) sym[i][j] = 0.5 * (g[i][j] + g[j][i]);
       const double tr = sym[0][0] + sym[1][1] + sym[2][2];
-      const double tmp = 2.0 * s.lambda * s.sc2[o] * tr;
+      const double tmp = 2.0 * M.lambda * s.sc2[o] * tr;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
           double geo = 0;
           for (int k = 0; k < 3; ++k) geo += g[i][k] * s.t2[o * 9 + k * 3 + j];
           P[i][j] = s.sc[o] * sym[i][j] + (i == j ? tmp : 0.0) + geo;
         }
-      for (int d = 0; d < 3; ++d) vs[d] = v[d] * s.rho_alpha * s.sc2[o];
+      for (int d = 0; d < 3; ++d) vs[d] = v[d] * M.rho_alpha * s.sc2[o];
     } else if (s.caching == 2) {
       // mf_nh_operator.h:976-994
       double sym[3][3];
@@ -429,7 +449,7 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
             for (int l = 0; l < 3; ++l) c4 += s.t4[o * 81 + ((i * 3 + j) * 3 + k) * 3 + l] * sym[k][l];
           P[i][j] = geo + c4;
         }
-      for (int d = 0; d < 3; ++d) vs[d] = v[d] * s.rho_alpha * s.sc2[o];
+      for (int d = 0; d < 3; ++d) vs[d] = v[d] * M.rho_alpha * s.sc2[o];
     } else if (s.caching == 3) {
       // mf_nh_operator.h:996-1010: double_contract<2,0,3,1>(A, g)_ij = A_ijkl g_kl
       for (int i = 0; i < 3; ++i)
@@ -439,7 +459,7 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
             for (int l = 0; l < 3; ++l) t += s.t4[o * 81 + ((i * 3 + j) * 3 + k) * 3 + l] * g[k][l];
           P[i][j] = t;
         }
-      for (int d = 0; d < 3; ++d) vs[d] = v[d] * s.rho_alpha;
+      for (int d = 0; d < 3; ++d) vs[d] = v[d] * M.rho_alpha;
     } else {
       // scalar_referential, mf_nh_operator.h:848-943
       double F[3][3], gx[3][3];
@@ -457,7 +477,7 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
           b[i][j] = t;
         }
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) tau[i][j] = s.mu * b[i][j] - (i == j ? s.sc[o] : 0.0);
+        for (int j = 0; j < 3; ++j) tau[i][j] = M.mu * b[i][j] - (i == j ? s.sc[o] : 0.0);
       inv3(F, Fi);
       for (int d = 0; d < 3; ++d)
         for (int e = 0; e < 3; ++e) {
@@ -469,7 +489,7 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
       const double trg = gx[0][0] + gx[1][1] + gx[2][2];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
-          jc[i][j] = 2.0 * s.sc[o] * 0.5 * (gx[i][j] + gx[j][i]) + (i == j ? 2.0 * s.lambda * trg : 0.0);
+          jc[i][j] = 2.0 * s.sc[o] * 0.5 * (gx[i][j] + gx[j][i]) + (i == j ? 2.0 * M.lambda * trg : 0.0);
       double queued[3][3];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
@@ -483,7 +503,7 @@ static void solid_cell(const Solid &s, int cell, const double *loc, double *out)
           for (int k = 0; k < 3; ++k) t += queued[i][k] * Fi[j][k];
           P[i][j] = t;
         }
-      for (int d = 0; d < 3; ++d) vs[d] = v[d] * s.rho_alpha;
+      for (int d = 0; d < 3; ++d) vs[d] = v[d] * M.rho_alpha;
     }
     // submit: P -> reference gradient  sum_e P[d][e] jinv[f][e] * JxW
     for (int d = 0; d < 3; ++d) {
@@ -605,6 +625,7 @@ static void shape_at_q(const Basis &B, int q, const double *jinv, vector<double>
 }
 
 static void solid_bruteforce_cell(const Solid &s, int cell, const double *loc, double *out) {
+  const Material &M = s.material(cell);
   // First Piola stress derivative of the one-field neo-Hookean (formulation 1):
   // dP_ij/dF_kl = mu d_ik d_jl + (mu - 2 lambda lnJ) Fi_li Fi_jk + 2 lambda Fi_ji Fi_lk
   const Mesh &m = *s.m;
@@ -626,14 +647,14 @@ static void solid_bruteforce_cell(const Solid &s, int cell, const double *loc, d
     const double J = det3(F);
     double Fi[3][3];
     inv3(F, Fi);
-    const double sc = s.mu - 2.0 * s.lambda * std::log(J);
+    const double sc = M.mu - 2.0 * M.lambda * std::log(J);
     double trFig = 0;
     for (int l = 0; l < 3; ++l)
       for (int k = 0; k < 3; ++k) trFig += Fi[l][k] * gv[k][l];
     double dP[3][3];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
-        double t = s.mu * gv[i][j] + 2.0 * s.lambda * Fi[j][i] * trFig;
+        double t = M.mu * gv[i][j] + 2.0 * M.lambda * Fi[j][i] * trFig;
         for (int k = 0; k < 3; ++k)
           for (int l = 0; l < 3; ++l) t += sc * Fi[l][i] * Fi[j][k] * gv[k][l];
         dP[i][j] = t;
@@ -641,7 +662,7 @@ static void solid_bruteforce_cell(const Solid &s, int cell, const double *loc, d
     const double w = s.G.jxw[o];
     for (int i = 0; i < N3; ++i)
       for (int d = 0; d < 3; ++d) {
-        double t = s.rho_alpha * v[d] * phi[i];
+        double t = M.rho_alpha * v[d] * phi[i];
         for (int e = 0; e < 3; ++e) t += dP[d][e] * dphi[i * 3 + e];
         out[d * N3 + i] += t * w;
       }
@@ -758,10 +779,20 @@ void *orc_solid_create(void *mesh, int n_q, double mu, double nu, double rho, do
   s->B.init(*s->m, n_q);
   s->G.init(*s->m, s->B);
   s->Q3 = n_q * n_q * n_q;
-  s->mu = mu; s->nu = nu; s->rho = rho; s->alpha1 = alpha_1;
+  s->alpha1 = alpha_1;
   s->form = formulation; s->caching = caching;
-  material_constants(*s);
+  s->mat[0] = s->mat[1] = material_constants(mu, nu, rho, alpha_1);
   return s;
+}
+void orc_solid_set_inclusion(void *op, const int32_t *material_ids, double mu, double nu, double rho) {
+  Solid &s = *(Solid *)op;
+  if (material_ids) {
+    s.mat_id.assign(material_ids, material_ids + s.m->n_cells);
+    s.mat[1] = material_constants(mu, nu, rho, s.alpha1);
+  } else {
+    s.mat_id.clear();
+    s.mat[1] = s.mat[0];
+  }
 }
 void orc_solid_destroy(void *op) { delete (Solid *)op; }
 int orc_solid_cache(void *op, const double *u) { return solid_cache(*(Solid *)op, u); }
@@ -955,6 +986,11 @@ void orc_solid_residual(void *op, const double *u, const double *acc, const doub
       for (int cell = 0; cell < m.n_cells; ++cell) {
         int cx = cell % m.n[0], cy = (cell / m.n[0]) % m.n[1], cz = cell / (m.n[0] * m.n[1]);
         if (((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2)) != colour) continue;
+#ifdef ORC_PLANT_INCL_RESIDUAL_MAT0
+        const Material &M = s.mat[0];  // planted: the residual ignores the cell's material
+#else
+        const Material &M = s.material(cell);  // mf_elasticity.h:1781-1784
+#endif
         gather(m, cell, u, false, lu);
         gather(m, cell, acc, false, la);
         eval_vector(s.B, 3, N3, lu.data(), uval, ugr);
@@ -965,9 +1001,9 @@ void orc_solid_residual(void *op, const double *u, const double *acc, const doub
           double F[3][3], Fi[3][3], tau[3][3];
           deformation_gradient(ugr.data(), Q3, q, jinv, F);
           inv3(F, Fi);
-          get_tau(s, F, tau);
+          get_tau(s, M, F, tau);
           for (int d = 0; d < 3; ++d) {
-            rv[(size_t)d * Q3 + q] = (-aval[(size_t)d * Q3 + q] + body[d]) * s.rho * s.G.jxw[o];
+            rv[(size_t)d * Q3 + q] = (-aval[(size_t)d * Q3 + q] + body[d]) * M.rho * s.G.jxw[o];
             for (int f = 0; f < 3; ++f) {
               // -(tau F^-T)[d][e] mapped to the reference gradient
               double t = 0;

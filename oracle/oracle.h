@@ -81,6 +81,14 @@ void orc_fill_uniform(uint64_t seed, int64_t n, double *v);
 void *orc_solid_create(void *mesh, int n_q, double mu, double nu, double rho,
                        double alpha_1, int formulation, int caching);
 void orc_solid_destroy(void *op);
+/* set_material's inclusion material (mf_nh_operator.h:121-128, 529-539):
+ * cells whose material_ids entry (one per cell, NULL: none) is not 0 use
+ * (mu, nu, rho), the others the material of orc_solid_create
+ * (mf_nh_operator.h:404, 658; residual mf_elasticity.h:1781-1784).
+ * [pinned: swapping the two materials cell by cell and adding gives the sum
+ * of the two one-material operators (and residuals); all cells of id 1 give
+ * the one-material operator of the inclusion; tests/test_oracle_operators.py] */
+void orc_solid_set_inclusion(void *op, const int32_t *material_ids, double mu, double nu, double rho);
 /* cache(): linearise at displacement u (mf_nh_operator.h:392-523). Returns 0,
  * or -1 if det F <= 0 at some quadrature point. */
 int orc_solid_cache(void *op, const double *u);

@@ -52,6 +52,7 @@ def lib():
             "orc_solid_create": (_vp, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_int, ctypes.c_int]),
             "orc_solid_destroy": (None, [_vp]),
+            "orc_solid_set_inclusion": (None, [_vp, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "orc_solid_cache": (ctypes.c_int, [_vp, _dp]),
             "orc_solid_vmult": (None, [_vp, _dp, _dp]),
             "orc_solid_vmult_bruteforce": (None, [_vp, _dp, _dp]),
@@ -231,13 +232,23 @@ class Mesh:
 class Solid:
     KIND = 0
 
-    def __init__(self, mesh, n_q=None, mu=MU, nu=NU, rho=RHO, alpha1=ALPHA1, formulation=1, caching=1):
+    def __init__(self, mesh, n_q=None, mu=MU, nu=NU, rho=RHO, alpha1=ALPHA1, formulation=1, caching=1,
+                 inclusion=None, material_ids=None):
+        """inclusion: (mu, nu, rho) of the material on the cells whose entry of material_ids (one per cell)
+        is not 0."""
         self.mesh = mesh
         n_q = mesh.p + 1 if n_q is None else n_q
         self.mu, self.nu, self.rho, self.alpha1 = mu, nu, rho, alpha1
         self.h = lib().orc_solid_create(mesh.h, n_q, mu, nu, rho, alpha1, formulation, caching)
         if not self.h:
             raise ValueError("unsupported solid operator configuration")
+        self.inclusion = None
+        if inclusion is not None and material_ids is not None:
+            self._ids = np.ascontiguousarray(material_ids, dtype=np.int32).reshape(-1)
+            assert len(self._ids) == mesh.n_cells
+            self.inclusion = tuple(inclusion)
+            lib().orc_solid_set_inclusion(self.h, self._ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          *self.inclusion)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -452,6 +463,8 @@ class SimdSolid:
     baseline of the benchmark; equals Solid.vmult)."""
 
     def __init__(self, solid):
+        if solid.inclusion is not None:
+            raise ValueError("the SIMD restatement takes one material")
         L, self.isa = _simd_lib()
         self.lanes = L.orc_simd_lanes()
         m = solid.mesh

@@ -118,9 +118,10 @@ def test_tangent_is_derivative_of_residual(form):
     assert np.linalg.norm(fd - Av) / np.linalg.norm(Av) < 1e-8
 
 
-def _numpy_energy(m, u, mu, lam):
+def _numpy_energy(m, u, mu, lam, cell_mat=None):
     """int Psi dX for Psi = mu/2 (tr C - 3 - 2 lnJ) + lam (lnJ)^2, evaluated in
-    numpy with its own tensor-product quadrature (one box cell layout)."""
+    numpy with its own tensor-product quadrature (one box cell layout);
+    cell_mat: optional (mu, lam) per cell instead."""
     p, N = m.p, m.p + 1
     z = O.gll_points(p)
     xq, wq = O.gauss_rule(N)
@@ -140,7 +141,8 @@ def _numpy_energy(m, u, mu, lam):
         F = np.moveaxis(F, (0, 1), (-2, -1))
         J = np.linalg.det(F)
         trC = np.einsum("...ij,...ij->...", F, F)
-        psi = mu / 2 * (trC - 3 - 2 * np.log(J)) + lam * np.log(J) ** 2
+        mc, lc = cell_mat[c] if cell_mat is not None else (mu, lam)
+        psi = mc / 2 * (trC - 3 - 2 * np.log(J)) + lc * np.log(J) ** 2
         W = np.einsum("k,j,i->kji", wq, wq, wq) * np.prod(h)
         E += np.sum(psi * W)
     return E
@@ -158,6 +160,95 @@ def test_residual_is_minus_energy_gradient():
         eps = 1e-6
         dE = (_numpy_energy(m, u + eps * v, O.MU, lam) - _numpy_energy(m, u - eps * v, O.MU, lam)) / (2 * eps)
         assert abs(dE + R @ v) < 1e-6 * abs(dE)
+
+
+# the inclusion material: the reference builds it with 100 mu (mf_elasticity.h:470-476); the density differs
+# too here, so that the inertia terms are checked as well
+INCL = (100 * O.MU, O.NU, 2 * O.RHO)
+
+
+def test_residual_is_minus_energy_gradient_two_materials():
+    """Cells of material id 1 carry the inclusion material: the residual is the
+    gradient of the energy with the material chosen per cell."""
+    m = O.Mesh([0, 0, 0], [1.0, 0.8, 0.6], [2, 1, 1], 2, 3, face_mask=0)
+    u = m.u0(amp=1e-1, noise=1e-2)
+    ids = np.array([0, 1], dtype=np.int32)
+    s = O.Solid(m, caching=1, inclusion=INCL, material_ids=ids)
+    lame = lambda mu, nu: 2 * mu * nu / (1 - 2 * nu)
+    cell_mat = [(O.MU, lame(O.MU, O.NU)), (INCL[0], lame(INCL[0], INCL[1]))]
+    R = s.residual(u, np.zeros_like(u))
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        v = rng.uniform(-1, 1, m.n_dofs)
+        eps = 1e-6
+        dE = (_numpy_energy(m, u + eps * v, 0, 0, cell_mat) - _numpy_energy(m, u - eps * v, 0, 0, cell_mat)) / (2 * eps)
+        assert abs(dE + R @ v) < 1e-6 * abs(dE)
+
+
+@pytest.mark.parametrize("form,caching", [(1, 1), (1, 0), (1, 2), (1, 3), (0, 1)])
+def test_inclusion_is_chosen_cell_by_cell(form, caching):
+    """Two materials by cell (set_material(material, material_inclusion),
+    mf_nh_operator.h:404, 658): swapping the materials cell by cell and adding
+    gives the sum of the two one-material operators (vmult, diagonal and
+    residual), and with every cell of id 1 the operator is the one-material
+    operator of the inclusion."""
+    m = O.Mesh([0, 0, 0], [1.0, 0.7, 0.5], [3, 2, 2], 2, 3)
+    u = m.u0(amp=5e-2, noise=1e-3)
+    ids = (np.random.default_rng(3).uniform(size=m.n_cells) < 0.4).astype(np.int32)
+    assert 0 < ids.sum() < m.n_cells
+    base = (O.MU, O.NU, O.RHO)
+
+    def make(mat, inclusion=None, material_ids=None):
+        s = O.Solid(m, mu=mat[0], nu=mat[1], rho=mat[2], formulation=form, caching=caching, inclusion=inclusion,
+                    material_ids=material_ids)
+        s.cache(u)
+        return s
+
+    A, B = make(base, INCL, ids), make(base, INCL, 1 - ids)
+    H0, H1 = make(base), make(INCL)
+    x = O.fill_uniform(11, m.n_dofs)
+    acc = O.fill_uniform(12, m.n_dofs)
+    body = (0.5, -9.81, 2.0)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(A.vmult(x) + B.vmult(x), H0.vmult(x) + H1.vmult(x)) < 1e-13
+    assert rel(A.diagonal() + B.diagonal(), H0.diagonal() + H1.diagonal()) < 1e-13
+    assert rel(A.residual(u, acc, body) + B.residual(u, acc, body),
+               H0.residual(u, acc, body) + H1.residual(u, acc, body)) < 1e-13
+    ones = make(base, INCL, np.ones(m.n_cells, dtype=np.int32))
+    assert rel(ones.vmult(x), H1.vmult(x)) < 1e-15
+    assert rel(ones.residual(u, acc, body), H1.residual(u, acc, body)) < 1e-15
+    # the two materials really differ on this mesh
+    assert rel(A.vmult(x), H0.vmult(x)) > 1e-2 and rel(A.vmult(x), H1.vmult(x)) > 1e-2
+
+
+INCLUSION_PINS = ["test_residual_is_minus_energy_gradient_two_materials", "test_inclusion_is_chosen_cell_by_cell"]
+
+
+@pytest.mark.parametrize("plant,caught_by", [
+    ("ORC_PLANT_INCL_CACHE_MAT0", ["test_inclusion_is_chosen_cell_by_cell"]),
+    ("ORC_PLANT_INCL_RESIDUAL_MAT0", INCLUSION_PINS),
+])
+def test_inclusion_pins_fail_on_a_planted_mistake(plant, caught_by, tmp_path):
+    """A copy of the oracle whose cache or residual ignores the cell's material
+    fails the inclusion pins above."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    odir = os.path.join(here, "..", "oracle")
+    lib = str(tmp_path / "liboracle_planted.so")
+    r = subprocess.run(["make", "-s", "-C", odir, "-B", f"LIB={lib}", f"PLANT=-D{plant}", lib], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    env = dict(os.environ, MFG_ORACLE_LIB=lib)
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-rf",
+                        os.path.join(here, "test_oracle_operators.py"), "-k", " or ".join(INCLUSION_PINS)],
+                       capture_output=True, text=True, env=env, cwd=here)
+    assert r.returncode != 0, r.stdout
+    failed = {line.split("::")[1].split("[")[0].split(" ")[0] for line in r.stdout.splitlines()
+              if line.startswith("FAILED")}
+    for name in caught_by:
+        assert name in failed, (name, r.stdout[-3000:])
 
 
 def _linear_elasticity_dense(m, mu, lam):
