@@ -295,7 +295,7 @@ def batched_bench(mfgpu, torch, reps=20, n_range=(4, 24)):
 
 def serving_bench(mfgpu, n_sessions=32, n_requests=64, rate=40.0, seed=100, max_concurrent=4):
     """32 live solid sessions (p ~ U{2..4}, beam [0,4]x[0,1]^2 with n_x ~ U{8,12,..,48}, n_y = n_z = n_x/4,
-    config-0 material, u0 = 1e-3 sin sin sin + 1e-5 noise); requests are one Newmark step each (Newton to
+    E = 1e6, nu = 0.3, rho = 1000, u0 = 1e-3 sin sin sin + 1e-5 noise); requests are one Newmark step each (Newton to
     ||R||/||R0|| <= 1e-8, MG-CG per linear solve) arriving as a Poisson process (seed 100).  Served with
     continuous batching (mixed-precision MG, up to 4 steps in flight, HBM-budget admission) against the same
     requests run one after another with plain FP64 MG."""

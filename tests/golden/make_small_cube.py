@@ -1,4 +1,4 @@
-"""Writes config0.npz: the oracle's outputs on the small FP64 configuration.
+"""Writes small_cube.npz: the oracle's outputs on the small FP64 configuration.
 
 Unit cube [0,1]^3, coarse 2x2x2 refined twice (8^3 cells, 3 levels), vector
 FE_Q(2) on Gauss-Lobatto nodes, QGauss(3); neo-Hookean E = 1e6, nu = 0.3,
@@ -9,7 +9,7 @@ A x = A x_true (x_true seed 2, constrained entries zero; Chebyshev degree 4,
 Chebyshev coarse solver) to ||r||/||b|| <= 1e-8: iterations, solution and
 the level eigenvalue estimates.
 
-Run from the repository root: python tests/golden/make_config0.py
+Run from the repository root: python tests/golden/make_small_cube.py
 """
 import hashlib
 import os
@@ -54,5 +54,5 @@ def compute():
 
 if __name__ == "__main__":
     out = compute()
-    np.savez_compressed(os.path.join(HERE, "config0.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, "small_cube.npz"), **out)
     print({k: (v if np.ndim(v) == 0 else np.shape(v)) for k, v in out.items()})

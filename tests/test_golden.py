@@ -1,5 +1,5 @@
-"""Committed fixtures of the small FP64 configuration (tests/golden/config0.npz,
-written by tests/golden/make_config0.py from the oracle).  The CPU tests pin
+"""Committed fixtures of the small FP64 configuration (tests/golden/small_cube.npz,
+written by tests/golden/make_small_cube.py from the oracle).  The CPU tests pin
 the oracle against the fixture (a change in the oracle's layout, operator or
 solver shows here); the GPU tests hold the device path to the same numbers:
 vmult to 1e-12, MG-CG iterations +-1 and solution to 1e-8."""
@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 
-FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config0.npz")
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "small_cube.npz")
 
 
 @pytest.fixture(scope="module")

@@ -33,11 +33,11 @@ def _lex_dofs(mesh_cellwise, nc):
 @pytest.fixture(scope="module")
 def instances(cuda):
     inst = bench.batched_instances(mfgpu)
-    print(f"config 3: {len(inst)} instances, {sum(i.n_dofs for i in inst)} dofs", flush=True)
+    print(f"48-instance batch: {len(inst)} instances, {sum(i.n_dofs for i in inst)} dofs", flush=True)
     return inst
 
 
-def test_config3_batched_vmult_full_size(instances):
+def test_batched_vmult_48_instances(instances):
     inst = instances
     srcs = [mfgpu.fill_uniform(500 + k, torch.empty(i.n_dofs, dtype=torch.float64, device="cuda"))
             for k, i in enumerate(inst)]
@@ -77,7 +77,7 @@ def test_config3_batched_vmult_full_size(instances):
         assert r <= 1e-12, (kind, p)
 
 
-def test_config3_batched_mgcg_full_size(instances):
+def test_batched_mgcg_48_instances(instances):
     inst = instances
     bs, xts = [], []
     for k, i in enumerate(inst):
@@ -98,7 +98,7 @@ def test_config3_batched_mgcg_full_size(instances):
         assert info.iterations == inf.iterations
         assert torch.equal(xq, x)
         assert float((x - xt).norm() / xt.norm()) <= 1e-5
-    print(f"config 3 MG-CG: iterations {[inf.iterations for inf in infos]}", flush=True)
+    print(f"48-instance batch MG-CG: iterations {[inf.iterations for inf in infos]}", flush=True)
 
 
 def _serving_workload():
@@ -118,11 +118,11 @@ def _serving_workload():
     return shapes, reqs, specs
 
 
-def test_config4_serving_full_size(cuda):
+def test_serving_32_sessions(cuda):
     shapes, reqs, specs = _serving_workload()
     alone, st1, fin1 = mfgpu.serve(specs, reqs, max_concurrent=1, real_time=False, final_states=True)
     mix, st4, fin4 = mfgpu.serve(specs, reqs, max_concurrent=4, real_time=False, final_states=True)
-    print(f"config 4: alone {st1.steps_per_s:.1f} steps/s, mix {st4.steps_per_s:.1f} steps/s "
+    print(f"32-session serving: alone {st1.steps_per_s:.1f} steps/s, mix {st4.steps_per_s:.1f} steps/s "
           f"(p50 {st4.latency_p50_s * 1e3:.1f} ms, p99 {st4.latency_p99_s * 1e3:.1f} ms)", flush=True)
     assert st1.completed == st4.completed == len(reqs) and st4.failed == 0
     for a, b in zip(alone, mix):
