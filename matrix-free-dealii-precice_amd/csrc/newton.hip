@@ -145,10 +145,27 @@ void SolidProblem::step(mfg_step_info &info) {
     }
     update_matrix_free();
     ws.ensure((size_t)N * 8);
-    if (nm.cond_cg_iterations > 0)  // solve_linear_system's condition estimate (mf_elasticity.h:1998-2013)
-      info.condition_estimate =
-          cg_condition_estimate(F, R.ptr, du.ptr, nm.cond_cg_iterations, nm.tol_lin * res,
-                                CGVectors{ws.r.ptr, ws.z.ptr, ws.p.ptr, ws.h.ptr}, red);
+    if (nm.cond_cg_iterations > 0) {
+      // solve_linear_system's condition estimate (mf_elasticity.h:1998-2013):
+      // 20 unpreconditioned CG steps that only report the Lanczos condition
+      // number.  With FP32 levels it runs on the FP32 fine-level operator, like
+      // the rest of the mixed-precision preconditioning work; the solution
+      // does not depend on it.
+      const CGVectors cv{ws.r.ptr, ws.z.ptr, ws.p.ptr, ws.h.ptr};
+      if (fine64) {
+        OperatorData &F32 = *levels.back();
+        if (cond_b.bytes != (size_t)N * 4) {
+          cond_b.allocate((size_t)N * 4);
+          cond_x.allocate((size_t)N * 4);
+        }
+        vec_convert(MFG_FP32, cond_b.ptr, MFG_FP64, R.ptr, N, stream);
+        info.condition_estimate =
+            cg_condition_estimate(F32, cond_b.ptr, cond_x.ptr, nm.cond_cg_iterations, nm.tol_lin * res, cv, red);
+      } else {
+        info.condition_estimate =
+            cg_condition_estimate(F, R.ptr, du.ptr, nm.cond_cg_iterations, nm.tol_lin * res, cv, red);
+      }
+    }
     vec_fill(MFG_FP64, du.ptr, N, 0.0, stream);
     mfg_solve_info si{};
     cg_solve(F, MFG_PRECOND_MG, mg.get(), 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
@@ -189,7 +206,7 @@ size_t SolidProblem::memory() const {
   if (fine64) b += fine64->memory_consumption() + fine64->vertices.bytes;
   for (auto &t : transfers) b += t->lattice.bytes;
   for (auto &x : u_lev) b += x.bytes;
-  b += ws.r.bytes + ws.z.bytes + ws.p.bytes + ws.h.bytes;
+  b += ws.r.bytes + ws.z.bytes + ws.p.bytes + ws.h.bytes + cond_b.bytes + cond_x.bytes;
   return b;
 }
 
@@ -206,7 +223,7 @@ size_t estimate_solid_problem_bytes(int n_levels, const mfg_level_desc *levels, 
     if (l == n_levels - 1) {
       b += 12 * ndof * 8;  // state, residual, CG workspace
       if (level_precision == MFG_FP32 && n_levels > 1) {
-        b += nqp * (18 * 8 + 4) + (size_t)d.n_cells * 24 * 8;
+        b += nqp * (18 * 8 + 4) + (size_t)d.n_cells * 24 * 8 + 2 * ndof * 4;  // + FP32 condition-estimate vectors
         if (nqp <= kGatherMaxPoints) b += gather_bytes(nqp, d.n_nodes, d.n_components, 8);
       }
     }

@@ -18,6 +18,8 @@ struct SolidProblem {
   mfg_newmark_params nm{};
   double a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
   DeviceBuffer u, du, delta_u, v, a, v_old, a_old, R;
+  // FP32 levels: the condition estimate's right-hand side and iterate in FP32
+  DeviceBuffer cond_b, cond_x;
   std::vector<DeviceBuffer> u_lev;
   hipStream_t stream = nullptr;
   CGWorkspace ws;
