@@ -407,8 +407,9 @@ typedef struct mfg_newmark_params {
   double tol_lin;   /* linear solve: ||r|| <= tol_lin ||R|| */
   int32_t max_lin;  /* CG iteration cap per linear solve */
   /* Condition estimate before every linear solve: this many plain CG
-   * iterations on the fine operator (stopping early at ||r|| <= tol_lin ||R||),
-   * their Lanczos condition number reported and the iterate discarded
+   * iterations on the fine operator (stopping early at ||r|| <= tol_lin ||R||;
+   * with FP32 levels on the FP32 fine-level operator), their Lanczos condition
+   * number reported and the iterate discarded
    * (solve_linear_system, mf_elasticity.h:1998-2013; the reference's default
    * is on with 20 iterations, parameter_handling.h:183-184).  0 skips it. */
   int32_t cond_cg_iterations;
