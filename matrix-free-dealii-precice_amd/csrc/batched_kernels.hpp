@@ -89,13 +89,8 @@ template <int P, typename Number, class QOp>
 __device__ inline void batch_task_cells(const QOp &qop, const BatchInst<Number> &in, const BatchVecs<Number> &v,
                                         const BatchTask &tk, int assign, Number *s_tab, Number *s_tile) {
   constexpr int N = P + 1;
-  const int t = threadIdx.x;
-  if (t < N * N) {
-    s_tab[t] = in.shape[t];
-    s_tab[N * N + t] = in.colloc[t];
-  }
-  cell_group<P, Number, QOp>(qop, in.cell_nodes, v.src, v.dst, tk.cell_begin, tk.cell_end, in.cpb, assign, s_tab,
-                             s_tab + N * N, s_tile);
+  cell_group<P, Number, QOp>(qop, in.cell_nodes, v.src, v.dst, tk.cell_begin, tk.cell_end, in.cpb, assign, in.shape,
+                             in.colloc, s_tab, s_tab + N * N, s_tile);
 }
 
 // KINDS: the instance kinds this kernel is compiled for (kind_bit mask).  A

@@ -811,10 +811,15 @@ __device__ inline void store_node(Number *p, const Number (&v)[NC]) {
   *reinterpret_cast<NodeValues<Number, NC> *>(p) = x;
 }
 
+// g_shape / g_colloc: the N x N 1D matrices in global memory, copied into
+// s_shape / s_colloc here.  The LDS tile and the two tables must not be in use
+// by other threads of the workgroup on entry (a caller that runs several
+// groups in sequence puts a barrier between them).
 template <int P, typename Number, class QOp, bool kToCells = false, bool kSpareSlots = false>
 __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ cell_nodes,
                                   const Number *__restrict__ src, Number *__restrict__ dst, int cell_begin,
-                                  int cell_end, int cpb, int assign, const Number *s_shape, const Number *s_colloc,
+                                  int cell_end, int cpb, int assign, const Number *__restrict__ g_shape,
+                                  const Number *__restrict__ g_colloc, Number *s_shape, Number *s_colloc,
                                   Number *s_tile) {
   constexpr int N = P + 1, N3 = N * N * N, NC = QOp::NC;
   const int t = threadIdx.x;
@@ -834,6 +839,15 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   const int64_t qidx = active ? (int64_t)cell * N3 + pt : 0;
 
   uint32_t entry = active ? cell_nodes[qidx] : 0u;
+  // the 1D matrices are loaded with the map and written to LDS just before
+  // the first barrier of the sweeps, which publishes them: no wait on these
+  // loads stands before the gather
+  const bool table_lane = t < N * N;
+  Number sh = 0, co = 0;
+  if (table_lane) {
+    sh = g_shape[t];
+    co = g_colloc[t];
+  }
   const int64_t node = entry & kNodeMaskDev;
   // a node's NC values are contiguous: one wide load (FP64, NC = 3: a 16- and
   // an 8-byte load instead of three) for all components, constrained ones
@@ -873,7 +887,11 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   // lanes included), which lets the per-cell tables come in as scalar loads
   const int qcell = cpb == 1 ? __builtin_amdgcn_readfirstlane(cell < cell_end ? cell : 0) : (active ? cell : 0);
   typename QOp::Data qd = qop.load(qidx, qcell, pt);
-  __syncthreads();
+  // read only after the barrier that follows the first puts of the sweeps
+  if (table_lane) {
+    s_shape[t] = sh;
+    s_colloc[t] = co;
+  }
   if constexpr (QOp::kNodalField) {
     // reference gradient of the cached nodal field (read plain: constrained
     // entries as stored), mf_nh_operator.h:873-877
@@ -926,11 +944,11 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
   __shared__ Number s_shape[N * N], s_colloc[N * N];
   __shared__ Number s_tile[CPB * (kSpare ? 5 : 3) * NC * N3];
   const int t = threadIdx.x;
-  if (t < N * N) {
-    s_shape[t] = a.shape[t];
-    s_colloc[t] = a.colloc[t];
-  }
-  // rows of constrained dofs: identity (mf_nh_operator.h:622-624)
+  const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
+  cell_group<P, Number, QOp, false, kSpare>(qop, a.cell_nodes, a.src, a.dst, first_cell, a.cell_end, CPB, assign,
+                                            a.shape, a.colloc, s_shape, s_colloc, s_tile);
+  // rows of constrained dofs: identity (mf_nh_operator.h:622-624).  The
+  // cells never store a constrained component, so these rows can follow them.
   if (a.fix_mode) {
     const int64_t g = (int64_t)blockIdx.x * NT + t;
     for (int64_t idx = g; idx < a.n_constrained; idx += (int64_t)gridDim.x * NT) {
@@ -939,9 +957,6 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
       else a.dst[dof] += a.src[dof];
     }
   }
-  const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
-  cell_group<P, Number, QOp, false, kSpare>(qop, a.cell_nodes, a.src, a.dst, first_cell, a.cell_end, CPB, assign, s_shape,
-                             s_colloc, s_tile);
 }
 
 // GATHER schedule, first launch: every cell at once, each point storing its
@@ -953,14 +968,9 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
   constexpr bool kSpare = spare_slots<P, CPB, Number, NC>();  // as apply_cells: the same arithmetic
   __shared__ Number s_shape[N * N], s_colloc[N * N];
   __shared__ Number s_tile[CPB * (kSpare ? 5 : 3) * NC * N3];
-  const int t = threadIdx.x;
-  if (t < N * N) {
-    s_shape[t] = a.shape[t];
-    s_colloc[t] = a.colloc[t];
-  }
   const int first_cell = a.cell_begin + xcd_remap(blockIdx.x, gridDim.x) * CPB;
-  cell_group<P, Number, QOp, true, kSpare>(qop, a.cell_nodes, a.src, contrib, first_cell, a.cell_end, CPB, 1, s_shape,
-                                   s_colloc, s_tile);
+  cell_group<P, Number, QOp, true, kSpare>(qop, a.cell_nodes, a.src, contrib, first_cell, a.cell_end, CPB, 1, a.shape,
+                                           a.colloc, s_shape, s_colloc, s_tile);
 }
 
 // GATHER schedule, second launch: one thread per node sums the contributions
