@@ -521,15 +521,22 @@ int64_t mfg_session_n_dofs(const mfg_session_spec *spec);
  * one instance.  Schedules: MFG_SCHEDULE_QUEUE, one persistent launch per
  * application (workgroups take tasks from an atomic ticket; a task waits for
  * its instance's previous colour), MFG_SCHEDULE_COLOUR, one launch per
- * colour, or MFG_SCHEDULE_STREAMS, every instance's own operator launches
+ * colour, MFG_SCHEDULE_STREAMS, every instance's own operator launches
  * spread over several streams (instances assigned by work, longest first), so
- * that small instances overlap.  The tuner times the three and the cells per
- * task of every (kind, degree) present, and keeps the fastest.  For the
- * batched solver, STREAMS runs every instance's own mfg_cg_solve /
- * mfg_mg_vmult on worker threads with a stream each. */
+ * that small instances overlap, or MFG_SCHEDULE_GATHER (batched vmult only):
+ * the instances of one (kind, degree) packed into units of similar work, each
+ * unit one launch over the cells of all its instances into their
+ * contribution buffers and one launch of the ordered per-node sums, units
+ * spread over several streams (each instance's GATHER tables and buffer are
+ * its operator's: a batch and a GATHER application of one of its operators on
+ * another stream must not overlap).  The tuner times them and the cells per task of every
+ * (kind, degree) present, and keeps the fastest.  For the batched solver,
+ * STREAMS runs every instance's own mfg_cg_solve / mfg_mg_vmult on worker
+ * threads with a stream each. */
 #define MFG_SCHEDULE_COLOUR 0
 #define MFG_SCHEDULE_QUEUE 1
 #define MFG_SCHEDULE_STREAMS 2
+#define MFG_SCHEDULE_GATHER 3
 #define MFG_BATCH_SOLID_F1 0 /* tensor2, formulation 1 */
 #define MFG_BATCH_SOLID_F0 1 /* tensor2, formulation 0 */
 #define MFG_BATCH_HEAT 2
@@ -541,6 +548,7 @@ typedef struct mfg_batch_tuning {
   double ms_colour, ms_queue;   /* one application of the whole batch */
   double ms_streams;            /* the same with MFG_SCHEDULE_STREAMS (batched solver: one
                                    preconditioner application of all instances) */
+  double ms_gather;             /* the same with MFG_SCHEDULE_GATHER (batched vmult) */
 } mfg_batch_tuning;
 typedef struct mfg_batch_t *mfg_batch;
 /* All operators must be cached and share one precision. */

@@ -40,7 +40,7 @@ struct Tuning {
   int cpb[8 * kNumKinds] = {};  // per code; 0 = the largest that fits kBatchThreads
   int queue = 1;     // schedule: MFG_SCHEDULE_COLOUR, _QUEUE or _STREAMS
   int waves = 2;     // register budget of the kernel: 2 or 4 waves per SIMD (FP32 only)
-  double ms_colour = 0, ms_queue = 0, ms_streams = 0;
+  double ms_colour = 0, ms_queue = 0, ms_streams = 0, ms_gather = 0;
   int cpb_for(int code) const {
     const int m = max_cpb(code & 7), c = cpb[code];
     return c > 0 && c <= m ? c : m;
@@ -603,16 +603,198 @@ struct TransferGroup {
   }
 };
 
+// ------------------------------------------------ batch GATHER schedule ---
+// The instances of one (kind, degree) are packed into units of about equal
+// work (a large instance alone, small ones together); a unit is one cells
+// launch over all its instances' cells (batched_cells_kernel) followed, on the
+// same stream, by one segmented per-node sum over its instances
+// (batched_gather_kernel).  Units are spread over streams by work, so the sums
+// of one unit overlap the cells of others.  Each instance's GATHER tables and
+// contribution buffer are its operator's own (built on first use, as
+// mfg_operator_set_cell_schedule builds them): a batch and a GATHER
+// application of one of its operators on another stream must not overlap.
+struct BatchGather {
+  struct Unit {
+    int code = 0;  // kind * 8 + degree
+    std::vector<int> inst;
+    DeviceBuffer tasks, segs, block_begin;
+    int n_tasks = 0;
+    int64_t blocks = 0;
+    double work = 0;
+  };
+  std::vector<OperatorData *> ops;
+  int prec = MFG_FP64;
+  std::vector<Unit> units;
+  StreamPool pool;
+  DeviceBuffer vecs;
+  std::vector<const void *> last_src;
+  std::vector<void *> last_dst;
+
+  template <typename Number>
+  void build_t(hipStream_t s) {
+    const int n = (int)ops.size();
+    double total = 0;
+    std::map<int, std::vector<int>> by_code;
+    for (int i = 0; i < n; ++i) {
+      OperatorData &op = *ops[i];
+      if (!op.contrib.ptr) op.build_gather();
+      by_code[code_of(op)].push_back(i);
+      total += apply_work(op);
+    }
+    // units of at most cap work (one instance at least), largest first; the
+    // floor (about 2^18 solid quadrature points) keeps small batches at one
+    // unit per (kind, degree), where launches rather than work set the time
+    const double cap = std::max(total / (2.0 * kBatchStreams), 5.0e6);
+    units.clear();
+    for (auto &kv : by_code) {
+      std::vector<int> v = kv.second;
+      std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return apply_work(*ops[a]) > apply_work(*ops[b]); });
+      Unit *u = nullptr;
+      for (int i : v) {
+        const double w = apply_work(*ops[i]);
+        if (!u || u->work + w > cap) {
+          units.emplace_back();
+          u = &units.back();
+          u->code = kv.first;
+        }
+        u->inst.push_back(i);
+        u->work += w;
+      }
+    }
+    std::vector<double> work;
+    for (Unit &u : units) {
+      std::vector<BatchTask> tv;
+      for (int i : u.inst) {
+        const OperatorData &op = *ops[i];
+        const int p = op.p;
+        const int cpb = p == 1 ? DefaultCPB<1, Number>::value : p == 2 ? DefaultCPB<2, Number>::value : 1;
+        for (int c = 0; c < op.n_cells; c += cpb) {
+          BatchTask t{};
+          t.inst = i;
+          t.cell_begin = c;
+          t.cell_end = std::min(op.n_cells, c + cpb);
+          tv.push_back(t);
+        }
+      }
+      u.n_tasks = (int)tv.size();
+      upload_vec(u.tasks, tv, s);
+      work.push_back(u.work);
+    }
+    pool.build(work, kBatchStreams);
+  }
+
+  BatchGather(const std::vector<OperatorData *> &o, hipStream_t s) : ops(o) {
+    prec = ops[0]->precision;
+    if (prec == MFG_FP64) build_t<double>(s);
+    else build_t<float>(s);
+  }
+
+  template <typename Number>
+  void set_vectors(const void *const *src, void *const *dst, hipStream_t s) {
+    const size_t n = ops.size();
+    std::vector<const void *> sv(src, src + n);
+    std::vector<void *> dv(dst, dst + n);
+    if (sv == last_src && dv == last_dst && vecs.ptr) return;
+    std::vector<BatchVecs<Number>> v(n);
+    for (size_t i = 0; i < n; ++i) {
+      MFG_REQUIRE(src[i] && dst[i], MFG_ERR_ARGUMENT, "null vector");
+      MFG_REQUIRE(src[i] != dst[i], MFG_ERR_ARGUMENT, "vmult needs distinct source and destination");
+      v[i].src = (const Number *)src[i];
+      v[i].dst = ops[i]->contrib.as<Number>();
+    }
+    upload_vec(vecs, v, s);
+    for (Unit &u : units) {
+      std::vector<GatherSeg<Number>> g;
+      std::vector<int64_t> bb(1, 0);  // first workgroup of every segment, and the grid size
+      for (int i : u.inst) {
+        const OperatorData &op = *ops[i];
+        GatherSeg<Number> e;
+        e.contrib = op.contrib.as<Number>();
+        e.offsets = op.gather_offsets.as<int32_t>();
+        e.points = op.gather_points.as<int32_t>();
+        e.cmask = op.gather_cmask.as<uint8_t>();
+        e.src = (const Number *)src[i];
+        e.dst = (Number *)dst[i];
+        e.n_nodes = op.n_nodes;
+        g.push_back(e);
+        bb.push_back(bb.back() + (op.n_nodes + kGatherThreads - 1) / kGatherThreads);
+      }
+      MFG_REQUIRE(bb.back() < INT32_MAX, MFG_ERR_ARGUMENT, "batch unit too large for one gather launch");
+      upload_vec(u.segs, g, s);
+      upload_vec(u.block_begin, bb, s);
+      u.blocks = bb.back();
+    }
+    last_src = sv;
+    last_dst = dv;
+  }
+
+  template <int P, int KIND, typename Number>
+  void launch_cells(const Unit &u, const void *inst_table, hipStream_t s) const {
+    constexpr int CPB = DefaultCPB<P, Number>::value;
+    BatchCellsArgs<Number> a;
+    a.inst = (const BatchInst<Number> *)inst_table;
+    a.tasks = u.tasks.as<BatchTask>();
+    a.vecs = vecs.as<BatchVecs<Number>>();
+    batched_cells_kernel<P, CPB, KIND, Number><<<u.n_tasks, block_threads<P, CPB>(), 0, s>>>(a);
+    MFG_HIP(hipGetLastError());
+  }
+
+  template <int KIND, typename Number>
+  void launch_kind(const Unit &u, const void *inst_table, hipStream_t s) const {
+    switch (u.code & 7) {
+      case 1: launch_cells<1, KIND, Number>(u, inst_table, s); break;
+      case 2: launch_cells<2, KIND, Number>(u, inst_table, s); break;
+      case 3: launch_cells<3, KIND, Number>(u, inst_table, s); break;
+      case 4: launch_cells<4, KIND, Number>(u, inst_table, s); break;
+      case 5: launch_cells<5, KIND, Number>(u, inst_table, s); break;
+      case 6: launch_cells<6, KIND, Number>(u, inst_table, s); break;
+      default: MFG_REQUIRE(false, MFG_ERR_ARGUMENT, "unsupported degree");
+    }
+  }
+
+  template <typename Number>
+  void apply_t(const void *inst_table, const void *const *src, void *const *dst, bool add, hipStream_t s) {
+    set_vectors<Number>(src, dst, s);
+    const int fix = add ? 2 : 1;
+    pool.run(s, [&](int ui, hipStream_t us) {
+      const Unit &u = units[ui];
+      const int kind = u.code >> 3;
+      switch (kind) {
+        case kSolidF1: launch_kind<kSolidF1, Number>(u, inst_table, us); break;
+        case kSolidF1G: launch_kind<kSolidF1G, Number>(u, inst_table, us); break;
+        case kSolidF0: launch_kind<kSolidF0, Number>(u, inst_table, us); break;
+        default: launch_kind<kHeat, Number>(u, inst_table, us); break;
+      }
+      const int n_segs = (int)u.inst.size();
+      if (kind == kHeat)
+        batched_gather_kernel<Number, 1><<<(unsigned)u.blocks, kGatherThreads, 0, us>>>(
+            u.segs.as<GatherSeg<Number>>(), u.block_begin.as<int64_t>(), n_segs, fix);
+      else
+        batched_gather_kernel<Number, 3><<<(unsigned)u.blocks, kGatherThreads, 0, us>>>(
+            u.segs.as<GatherSeg<Number>>(), u.block_begin.as<int64_t>(), n_segs, fix);
+      MFG_HIP(hipGetLastError());
+    });
+  }
+
+  void apply(const void *inst_table, const void *const *src, void *const *dst, bool add, hipStream_t s) {
+    if (prec == MFG_FP64) apply_t<double>(inst_table, src, dst, add, s);
+    else apply_t<float>(inst_table, src, dst, add, s);
+  }
+};
+
 // ---------------------------------------------------- batched vmult API ---
 struct BatchVmult {
   std::unique_ptr<ApplyGroup> group;
+  std::vector<OperatorData *> mops;  // the same operators (the GATHER schedule builds their tables)
+  std::unique_ptr<BatchGather> gather;
   Tuning tu;
   hipStream_t stream = nullptr;
   DeviceBuffer vecs, error;
   std::vector<const void *> last_src;
   std::vector<void *> last_dst;
 
-  BatchVmult(const std::vector<const OperatorData *> &ops) {
+  BatchVmult(const std::vector<OperatorData *> &mo) : mops(mo) {
+    const std::vector<const OperatorData *> ops(mo.begin(), mo.end());
     std::vector<int> ids(ops.size());
     for (size_t i = 0; i < ids.size(); ++i) ids[i] = (int)i;
     group.reset(new ApplyGroup(ops, ids));
@@ -647,32 +829,53 @@ struct BatchVmult {
     }
     pool.run(stream, [&](int i, hipStream_t s) { group->ops[i]->apply(src[i], dst[i], add, s); });
   }
+  void apply_gather(const void *const *src, void *const *dst, bool add) {
+    if (!gather) gather.reset(new BatchGather(mops, stream));
+    gather->apply(group->inst.ptr, src, dst, add, stream);
+  }
   void apply(const void *const *src, void *const *dst, bool add) {
     if (tu.queue == MFG_SCHEDULE_STREAMS) {
       apply_streams(src, dst, add);
       return;
     }
+    if (tu.queue == MFG_SCHEDULE_GATHER) {
+      apply_gather(src, dst, add);
+      return;
+    }
     set_vectors(src, dst);
     group->launch(stream, vecs.ptr, nullptr, add, tu, error.as<int>());
   }
-  // the fused schedules' tuning, then the STREAMS schedule timed against the best of them
+  // the fused schedules' tuning, then the STREAMS and GATHER schedules timed
+  // against the best of them
   void tune(const void *const *src, void *const *dst, int reps) {
     set_vectors(src, dst);
     group->tune(tu, vecs.ptr, reps, stream, error.as<int>());
     hipEvent_t e0, e1;
     MFG_HIP(hipEventCreate(&e0));
     MFG_HIP(hipEventCreate(&e1));
-    for (int w = 0; w < 2; ++w) apply_streams(src, dst, false);
-    MFG_HIP(hipEventRecord(e0, stream));
-    for (int r = 0; r < reps; ++r) apply_streams(src, dst, false);
-    MFG_HIP(hipEventRecord(e1, stream));
-    MFG_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    MFG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    auto time_it = [&](auto &&fn) {
+      for (int w = 0; w < 2; ++w) fn();
+      MFG_HIP(hipEventRecord(e0, stream));
+      for (int r = 0; r < reps; ++r) fn();
+      MFG_HIP(hipEventRecord(e1, stream));
+      MFG_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      MFG_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / reps;
+    };
+    try {
+      tu.ms_streams = time_it([&] { apply_streams(src, dst, false); });
+      tu.ms_gather = time_it([&] { apply_gather(src, dst, false); });
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    tu.ms_streams = (double)ms / reps;
-    if (tu.ms_streams < std::min(tu.ms_colour, tu.ms_queue)) tu.queue = MFG_SCHEDULE_STREAMS;
+    const double fused = std::min(tu.ms_colour, tu.ms_queue);
+    if (tu.ms_streams < fused && tu.ms_streams <= tu.ms_gather) tu.queue = MFG_SCHEDULE_STREAMS;
+    else if (tu.ms_gather < fused) tu.queue = MFG_SCHEDULE_GATHER;
   }
   void synchronize() {
     MFG_HIP(hipStreamSynchronize(stream));
@@ -1374,6 +1577,7 @@ static void export_tuning(const Tuning &tu, mfg_batch_tuning *out) {
   out->ms_colour = tu.ms_colour;
   out->ms_queue = tu.ms_queue;
   out->ms_streams = tu.ms_streams;
+  out->ms_gather = tu.ms_gather;
 }
 
 extern "C" {
@@ -1381,7 +1585,7 @@ extern "C" {
 int mfg_batch_create(int32_t n, const mfg_operator *ops, mfg_batch *out) {
   return guarded([&] {
     MFG_REQUIRE(out && ops && n > 0, MFG_ERR_ARGUMENT, "null argument");
-    std::vector<const OperatorData *> v;
+    std::vector<OperatorData *> v;
     for (int i = 0; i < n; ++i) {
       MFG_REQUIRE(ops[i], MFG_ERR_ARGUMENT, "null operator");
       v.push_back(ops[i]);
@@ -1407,7 +1611,7 @@ int mfg_batch_set_schedule(mfg_batch b, int32_t schedule) {
   return guarded([&] {
     MFG_REQUIRE(b, MFG_ERR_ARGUMENT, "null batch");
     MFG_REQUIRE(schedule == MFG_SCHEDULE_COLOUR || schedule == MFG_SCHEDULE_QUEUE ||
-                    schedule == MFG_SCHEDULE_STREAMS,
+                    schedule == MFG_SCHEDULE_STREAMS || schedule == MFG_SCHEDULE_GATHER,
                 MFG_ERR_ARGUMENT, "unknown schedule");
     b->b->tu.queue = schedule;
   });

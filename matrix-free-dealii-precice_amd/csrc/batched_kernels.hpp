@@ -201,6 +201,82 @@ __global__ void __launch_bounds__(kBatchThreads, WAVES) batched_apply_kernel(Bat
   }
 }
 
+// ------------------------------------------------------ GATHER schedule ---
+// One launch per (kind, degree) present: every cell of every instance of that
+// kind and degree, a workgroup per task (DefaultCPB cells of one instance),
+// with the register allocation and the cell arithmetic of the single-operator
+// kernels.  Each point stores its contribution into its instance's
+// contribution buffer: no colours and no waits between workgroups.  Then
+// batched_gather_kernel sums every node's contributions in colour order, the
+// order in which the colour launches add them, so the result is the
+// per-instance result bit for bit.
+template <int KIND, bool kEarlyCell, typename Number>
+__device__ inline auto batch_qop(const BatchInst<Number> &in) {
+  if constexpr (KIND == kSolidF1) {
+    return QSolidT2F1<Number>{in.qd, in.nqp, in.prm[0], in.prm[1], in.cell_jxw, in.wq};
+  } else if constexpr (KIND == kSolidF1G) {
+    return QSolidT2F1G<Number, kEarlyCell>{in.qd,       in.nqp,      in.prm[2], in.prm[0],
+                                           in.prm[1],   in.cell_jac, in.cell_jxw, in.wq};
+  } else if constexpr (KIND == kSolidF0) {
+    return QSolidT2F0<Number>{in.qd, in.nqp, in.prm[0], in.prm[1], in.prm[2]};
+  } else {
+    return QHeat<Number>{in.qd, in.nqp, in.prm[0], in.cell_geo, in.wq};
+  }
+}
+
+template <typename Number>
+struct BatchCellsArgs {
+  const BatchInst<Number> *inst;
+  const BatchTask *tasks;         // this launch's tasks: inst, cell_begin, cell_end
+  const BatchVecs<Number> *vecs;  // per instance: src and the instance's contribution buffer
+};
+
+template <int P, int CPB, int KIND, typename Number>
+__global__ void __launch_bounds__((block_threads<P, CPB>())) batched_cells_kernel(BatchCellsArgs<Number> a) {
+  constexpr int N = P + 1, N3 = N * N * N, NC = KIND == kHeat ? 1 : 3;
+  constexpr bool kSpare = spare_slots<P, CPB, Number, NC>();  // as apply_cells: the same arithmetic
+  __shared__ Number s_shape[N * N], s_colloc[N * N];
+  __shared__ Number s_tile[CPB * (kSpare ? 5 : 3) * NC * N3];
+  const BatchTask tk = a.tasks[xcd_remap(blockIdx.x, gridDim.x)];
+  const BatchInst<Number> in = a.inst[tk.inst];
+  const BatchVecs<Number> v = a.vecs[tk.inst];
+  const auto q = batch_qop<KIND, CPB == 1>(in);
+  cell_group<P, Number, decltype(q), true, kSpare>(q, in.cell_nodes, v.src, v.dst, tk.cell_begin, tk.cell_end, CPB,
+                                                   1, in.shape, in.colloc, s_shape, s_colloc, s_tile);
+}
+
+template <typename Number>
+struct GatherSeg {
+  const Number *contrib;
+  const int32_t *offsets, *points;
+  const uint8_t *cmask;
+  const Number *src;
+  Number *dst;
+  int64_t n_nodes;
+};
+
+constexpr int kGatherThreads = 256;
+
+// One workgroup per kGatherThreads nodes of one instance (segment);
+// block_begin[s] is segment s's first workgroup (n_segs + 1 entries, the last
+// one the grid size).
+template <typename Number, int NC>
+__global__ void __launch_bounds__(kGatherThreads)
+    batched_gather_kernel(const GatherSeg<Number> *__restrict__ segs, const int64_t *__restrict__ block_begin,
+                          int n_segs, int fix_mode) {
+  const int64_t blk = blockIdx.x;
+  int lo = 0, hi = n_segs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (block_begin[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const GatherSeg<Number> g = segs[lo];
+  const int64_t node = (blk - block_begin[lo]) * kGatherThreads + threadIdx.x;
+  if (node >= g.n_nodes) return;
+  gather_node<Number, NC>(g.contrib, g.offsets, g.points, g.cmask, g.src, g.dst, node, fix_mode);
+}
+
 // --------------------------------------------------- segmented vectors ---
 // One segment per instance vector; a workgroup handles one chunk of
 // kSegChunk entries of one segment (found by binary search on chunk_begin).

@@ -33,6 +33,17 @@ constexpr int block_threads() {
   return ((CPB * (P + 1) * (P + 1) * (P + 1) + 63) / 64) * 64;
 }
 
+// cells per workgroup of the apply / cache kernels, per degree.  Q4: one cell
+// per workgroup measured 3-5 % faster than two for the solid and the heat
+// operator in FP64 and, once the per-cell tables became scalar loads, 2-5 %
+// in FP32.  Q3 and Q5 with one cell instead of two, Q2 with two cells (one
+// wave) instead of seven, on 3-6M-DoF meshes: solid FP64 -4 to -30 %, heat
+// -4 to -26 %, FP32 -1 to -3 %, small meshes unchanged.
+template <int P, typename Number>
+struct DefaultCPB {
+  static constexpr int value = P == 1 ? 8 : P == 2 ? 2 : 1;
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
 // XCD walks one contiguous range of cells (shared nodes stay in one L2).
 __device__ inline int xcd_remap(int b, int G) {
@@ -980,12 +991,10 @@ __global__ void __launch_bounds__((block_threads<P, CPB>()))
 // overwritten; 2 (vmult_add): constrained components add src, the others add
 // to dst.
 template <typename Number, int NC>
-__global__ void gather_contributions(const Number *__restrict__ contrib, const int32_t *__restrict__ offsets,
-                                     const int32_t *__restrict__ points, const uint8_t *__restrict__ cmask,
-                                     const Number *__restrict__ src, Number *__restrict__ dst, int64_t n_nodes,
-                                     int fix_mode) {
-  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= n_nodes) return;
+__device__ inline void gather_node(const Number *__restrict__ contrib, const int32_t *__restrict__ offsets,
+                                   const int32_t *__restrict__ points, const uint8_t *__restrict__ cmask,
+                                   const Number *__restrict__ src, Number *__restrict__ dst, int64_t node,
+                                   int fix_mode) {
   const int b = offsets[node], e = offsets[node + 1];
   const unsigned m = cmask[node];
 #pragma unroll
@@ -1006,6 +1015,16 @@ __global__ void gather_contributions(const Number *__restrict__ contrib, const i
     for (; k < e; ++k) acc = acc + contrib[(int64_t)points[k] * NC + c];
     dst[dof] = acc;
   }
+}
+
+template <typename Number, int NC>
+__global__ void gather_contributions(const Number *__restrict__ contrib, const int32_t *__restrict__ offsets,
+                                     const int32_t *__restrict__ points, const uint8_t *__restrict__ cmask,
+                                     const Number *__restrict__ src, Number *__restrict__ dst, int64_t n_nodes,
+                                     int fix_mode) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= n_nodes) return;
+  gather_node<Number, NC>(contrib, offsets, points, cmask, src, dst, node, fix_mode);
 }
 
 }  // namespace mfg

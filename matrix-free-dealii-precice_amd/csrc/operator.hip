@@ -16,17 +16,6 @@ using namespace mfg;
 
 namespace mfg {
 
-// cells per workgroup of the apply / cache kernels, per degree.  Q4: one cell
-// per workgroup measured 3-5 % faster than two for the solid and the heat
-// operator in FP64 and, once the per-cell tables became scalar loads, 2-5 %
-// in FP32.  Q3 and Q5 with one cell instead of two, Q2 with two cells (one
-// wave) instead of seven, on 3-6M-DoF meshes: solid FP64 -4 to -30 %, heat
-// -4 to -26 %, FP32 -1 to -3 %, small meshes unchanged.
-template <int P, typename Number>
-struct DefaultCPB {
-  static constexpr int value = P == 1 ? 8 : P == 2 ? 2 : 1;
-};
-
 // per-cell reference JxW of the tensor2 formulation-1 op, or null when it is
 // a per-point field
 template <typename Number>

@@ -771,21 +771,23 @@ class SolidProblem:
 
 
 # ---------------------------------------------------- batched instances ---
-SCHEDULE_COLOUR, SCHEDULE_QUEUE, SCHEDULE_STREAMS = 0, 1, 2
+SCHEDULE_COLOUR, SCHEDULE_QUEUE, SCHEDULE_STREAMS, SCHEDULE_GATHER = 0, 1, 2, 3
 BATCH_SOLID_F1, BATCH_SOLID_F0, BATCH_HEAT, BATCH_SOLID_F1_GRADIENT = 0, 1, 2, 3
 
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int32), ("cells_per_task", (ctypes.c_int32 * 7) * 4),
                 ("waves_per_simd", ctypes.c_int32), ("ms_colour", ctypes.c_double), ("ms_queue", ctypes.c_double),
-                ("ms_streams", ctypes.c_double)]
+                ("ms_streams", ctypes.c_double), ("ms_gather", ctypes.c_double)]
 
     def as_dict(self):
         kinds = ("solid_f1", "solid_f0", "heat", "solid_f1_gradient")
-        return {"schedule": {SCHEDULE_COLOUR: "colour", SCHEDULE_QUEUE: "queue", SCHEDULE_STREAMS: "streams"}[self.schedule],
+        names = {SCHEDULE_COLOUR: "colour", SCHEDULE_QUEUE: "queue", SCHEDULE_STREAMS: "streams",
+                 SCHEDULE_GATHER: "gather"}
+        return {"schedule": names[self.schedule],
                 "cells_per_task": {k: list(self.cells_per_task[i])[1:] for i, k in enumerate(kinds)},
                 "waves_per_simd": self.waves_per_simd, "ms_colour": self.ms_colour, "ms_queue": self.ms_queue,
-                "ms_streams": self.ms_streams}
+                "ms_streams": self.ms_streams, "ms_gather": self.ms_gather}
 
 
 _vpp = ctypes.POINTER(_vp)

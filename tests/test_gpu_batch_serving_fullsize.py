@@ -45,11 +45,13 @@ def test_batched_vmult_48_instances(instances):
     for i, s, r in zip(inst, srcs, ref):
         i.fine.vmult(r, s)
     b = mfgpu.Batch([i.fine for i in inst])
-    for rep in range(2):
+    # GATHER twice: the second call with other dst vectors rebuilds its tables
+    for sched in (mfgpu.SCHEDULE_QUEUE, mfgpu.SCHEDULE_STREAMS, mfgpu.SCHEDULE_GATHER, mfgpu.SCHEDULE_GATHER):
+        b.set_schedule(sched)
         out = [torch.full_like(s, float("nan")) for s in srcs]
         b.vmult(out, srcs)
         torch.cuda.synchronize()
-        assert all(torch.equal(o, r) for o, r in zip(out, ref)), rep
+        assert all(torch.equal(o, r) for o, r in zip(out, ref)), sched
     # the smallest instance of every (kind, degree) against the oracle
     strata = {}
     for k, i in enumerate(inst):

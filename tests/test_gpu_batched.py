@@ -45,7 +45,8 @@ SPECS = [("heat", [3, 4, 2], 1, 1), ("solid", [2, 3, 2], 2, 1), ("heat", [4, 2, 
 
 @pytest.mark.parametrize("precision,storage", [(mfgpu.FP64, mfgpu.T2_FOLDED), (mfgpu.FP32, mfgpu.T2_FOLDED),
                                                (mfgpu.FP64, mfgpu.T2_GRADIENT)])
-@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE, mfgpu.SCHEDULE_STREAMS])
+@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_QUEUE, mfgpu.SCHEDULE_STREAMS,
+                                      mfgpu.SCHEDULE_GATHER])
 def test_batched_vmult_equals_per_instance(cuda, precision, storage, schedule):
     print(f"batched vmult precision={precision} storage={storage} schedule={schedule}", flush=True)
     ops, meshes = make_ops(SPECS, precision, storage)
@@ -94,8 +95,8 @@ def test_batched_cells_per_task_and_tuner(cuda):
     t = b.tune(srcs, out, repetitions=2)
     d = t.as_dict()
     print("tuning", d)
-    assert d["schedule"] in ("queue", "colour", "streams") and t.waves_per_simd in (2, 4)
-    assert t.ms_streams > 0
+    assert d["schedule"] in ("queue", "colour", "streams", "gather") and t.waves_per_simd in (2, 4)
+    assert t.ms_streams > 0 and t.ms_gather > 0
     for p in range(1, 7):
         assert 1 <= d["cells_per_task"]["heat"][p - 1] <= 512 // (p + 1) ** 3
     out = [torch.empty_like(s) for s in srcs]
