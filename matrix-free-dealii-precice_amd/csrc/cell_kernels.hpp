@@ -326,8 +326,14 @@ struct QSolidT2F1 {
 // workgroup): the cell's J_ref, J_ref^{-1} and det J_ref are loaded into
 // SGPRs with the point data, so that their latency overlaps the gather's
 // instead of following the evaluate sweeps.
-template <typename Number, bool kEarlyCell = false>
+// kCartesian (with kEarlyCell; every cell's J_ref is diagonal, as on
+// axis-aligned boxes): only the 3 + 3 diagonal entries are loaded, and the
+// products with the exact zeros off the diagonal are left out.  Every
+// rounding step of the general form survives, so the two agree bit for bit
+// (up to the sign of a zero).
+template <typename Number, bool kEarlyCell = false, bool kCartesian = false>
 struct QSolidT2F1G {
+  static_assert(kEarlyCell || !kCartesian, "the Cartesian form loads the cell's diagonal into SGPRs");
   static constexpr int NC = 3;
   static constexpr int kFields = 10;
   static constexpr bool kNodalField = false;
@@ -340,7 +346,8 @@ struct QSolidT2F1G {
   struct Data {
     Number gu[9], s, w;
     const Number *jac;
-    Number jr[kEarlyCell ? 18 : 1];
+    // kCartesian: diag J_ref (3), diag J_ref^{-1} (3); else J_ref (9), J_ref^{-1} (9)
+    Number jr[kEarlyCell ? (kCartesian ? 6 : 18) : 1];
   };
   // fields are stored in pairs, qd[((f / 2) nqp + q) 2 + f % 2]: one 16-byte
   // (FP64) load brings two fields of a point
@@ -365,7 +372,17 @@ struct QSolidT2F1G {
   // the per-cell part of load(): J_ref, J_ref^{-1} and JxW_ref at point pt
   __device__ void load_cell(Data &d, int cell, int pt) const {
     d.jac = cell_jac + (int64_t)cell * 18;
-    if constexpr (kEarlyCell) {
+    if constexpr (kCartesian) {
+      Number jxw = as_constant(cell_jxw)[cell];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        d.jr[e] = as_constant(d.jac)[4 * e];
+        d.jr[3 + e] = as_constant(d.jac)[9 + 4 * e];
+      }
+      asm volatile("" : "+s"(jxw), "+s"(d.jr[0]), "+s"(d.jr[1]), "+s"(d.jr[2]), "+s"(d.jr[3]), "+s"(d.jr[4]),
+                   "+s"(d.jr[5]));
+      d.w = jxw * wq[pt];
+    } else if constexpr (kEarlyCell) {
       Number jxw = as_constant(cell_jxw)[cell];
 #pragma unroll
       for (int f = 0; f < 18; ++f) d.jr[f] = as_constant(d.jac)[f];
@@ -389,8 +406,13 @@ struct QSolidT2F1G {
     Number ji[9];
     {
       Number Jc[3][3];
+      if constexpr (kCartesian) {
 #pragma unroll
-      for (int f = 0; f < 9; ++f) Jc[f / 3][f % 3] = jr(f) + d.gu[f];
+        for (int f = 0; f < 9; ++f) Jc[f / 3][f % 3] = f % 4 == 0 ? d.jr[f / 4] + d.gu[f] : d.gu[f];
+      } else {
+#pragma unroll
+        for (int f = 0; f < 9; ++f) Jc[f / 3][f % 3] = jr(f) + d.gu[f];
+      }
       const Number id =
           Number(1) / (Jc[0][0] * (Jc[1][1] * Jc[2][2] - Jc[1][2] * Jc[2][1]) -
                        Jc[0][1] * (Jc[1][0] * Jc[2][2] - Jc[1][2] * Jc[2][0]) + Jc[0][2] * (Jc[1][0] * Jc[2][1] - Jc[1][1] * Jc[2][0]));
@@ -407,12 +429,24 @@ struct QSolidT2F1G {
     Number T[6];
     {
       Number F[3][3];
+      if constexpr (kCartesian) {
+        // the general form's sum of three products, two of them exact zeros,
+        // rounds as this one product does
 #pragma unroll
-      for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int e = 0; e < 3; ++e)
-          F[a][e] = d.gu[a * 3 + 0] * jr(9 + 0 * 3 + e) + d.gu[a * 3 + 1] * jr(9 + 1 * 3 + e) + d.gu[a * 3 + 2] * jr(9 + 2 * 3 + e) +
-                    (a == e ? Number(1) : Number(0));
+          for (int e = 0; e < 3; ++e) {
+            const Number t = d.gu[a * 3 + e] * d.jr[3 + e];
+            F[a][e] = a == e ? t + Number(1) : t;
+          }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+            F[a][e] = d.gu[a * 3 + 0] * jr(9 + 0 * 3 + e) + d.gu[a * 3 + 1] * jr(9 + 1 * 3 + e) +
+                      d.gu[a * 3 + 2] * jr(9 + 2 * 3 + e) + (a == e ? Number(1) : Number(0));
+      }
 #pragma unroll
       for (int s6 = 0; s6 < 6; ++s6) {
         const int x = kSymI[s6], y = kSymJ[s6];
@@ -879,7 +913,8 @@ __device__ inline void cell_group(const QOp &qop, const uint32_t *__restrict__ c
   // count it does not change, and elsewhere the early read is.
   constexpr bool kLateDst = sizeof(Number) == 8 && P <= 4 && !std::is_same<QOp, QHeat<Number>>::value &&
                             !std::is_same<QOp, QSolidT2F1G<Number>>::value &&
-                            !std::is_same<QOp, QSolidT2F1G<Number, true>>::value;
+                            !std::is_same<QOp, QSolidT2F1G<Number, true>>::value &&
+                            !std::is_same<QOp, QSolidT2F1G<Number, true, true>>::value;
   const bool first = assign && (entry >> 31);
   Number dold[NC];
   if constexpr (!kLateDst && !kToCells) {

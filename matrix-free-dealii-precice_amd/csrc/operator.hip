@@ -25,13 +25,13 @@ const Number *t2_cell_jxw(const OperatorData &op) {
 
 // tensor2 formulation 1 from the 10-word storage (affine meshes only), with
 // the constants of material m
-template <typename Number, bool kEarlyCell = false>
-QSolidT2F1G<Number, kEarlyCell> t2_gradient_op(const OperatorData &op, int m = 0) {
+template <typename Number, bool kEarlyCell = false, bool kCartesian = false>
+QSolidT2F1G<Number, kEarlyCell, kCartesian> t2_gradient_op(const OperatorData &op, int m = 0) {
   const OperatorData::MaterialConstants &M = op.mats[m];
-  return QSolidT2F1G<Number, kEarlyCell>{op.qd.as<Number>(),        op.nqp,
-                                         Number(M.mu),              Number(2.0 * M.lambda),
-                                         Number(M.rho_alpha),       op.cell_jac.as<Number>(),
-                                         op.cell_jxw.as<Number>(),  op.wq3.as<Number>()};
+  return QSolidT2F1G<Number, kEarlyCell, kCartesian>{op.qd.as<Number>(),       op.nqp,
+                                                     Number(M.mu),             Number(2.0 * M.lambda),
+                                                     Number(M.rho_alpha),      op.cell_jac.as<Number>(),
+                                                     op.cell_jxw.as<Number>(), op.wq3.as<Number>()};
 }
 
 // per-cell geometry of the heat op on affine meshes, or null (per-point fields)
@@ -42,7 +42,8 @@ const Number *heat_cell_geo(const OperatorData &op) {
 
 // fn(q) with the quadrature op of this operator for the cells of material m
 // (the heat op has one).  kEarlyCell: one cell per workgroup, the 10-word op
-// takes its per-cell tables into SGPRs.
+// takes its per-cell tables into SGPRs (only their diagonals when every
+// cell's reference Jacobian is diagonal).
 template <typename Number, bool kEarlyCell, class F>
 static void with_qop(const OperatorData &op, int m, F fn) {
   if (op.kind == 1) {
@@ -58,6 +59,12 @@ static void with_qop(const OperatorData &op, int m, F fn) {
   } else if (op.sp.caching == MFG_CACHING_TENSOR4_NS) {
     fn(QSolidT4NS<Number>{op.qd.as<Number>(), op.nqp, ra});
   } else if (op.t2_gradient()) {
+    if constexpr (kEarlyCell) {
+      if (op.cartesian_ref) {
+        fn(t2_gradient_op<Number, true, true>(op, m));
+        return;
+      }
+    }
     fn(t2_gradient_op<Number, kEarlyCell>(op, m));
   } else if (op.sp.formulation == 1) {
     fn(QSolidT2F1<Number>{op.qd.as<Number>(), op.nqp, Number(2.0 * M.lambda), ra, t2_cell_jxw<Number>(op),
@@ -504,6 +511,10 @@ void OperatorData::setup_layout(const mfg_level_desc &d) {
     G[16] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
     G[17] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
   }
+  // axis-aligned cells: J_ref and J_ref^{-1} exactly zero off the diagonal
+  cartesian_ref = affine_ref;
+  for (size_t x = 0; x < jac.size() && cartesian_ref; ++x)
+    if ((x % 9) % 4 != 0 && jac[x] != 0.0) cartesian_ref = false;
   upload(cell_nodes, map.data(), map.size(), stream);
   upload(constrained, d.constrained_dofs, (size_t)n_constrained, stream);
   host_constrained.assign(d.constrained_dofs, d.constrained_dofs + n_constrained);

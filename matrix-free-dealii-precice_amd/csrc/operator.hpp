@@ -43,6 +43,7 @@ struct OperatorData {
   DeviceBuffer cell_jxw, wq3;
   DeviceBuffer cell_geo;  // heat on affine cells: Jinv (9), det J per stored cell
   DeviceBuffer cell_jac;  // solid on affine cells: J_ref (9), J_ref^{-1} (9) per stored cell
+  bool cartesian_ref = false;  // affine, and every cell's J_ref (so J_ref^{-1}) is diagonal
   int t2_storage = MFG_T2_FOLDED;  // tensor2 formulation-1 point storage (MFG_T2_*)
   bool t2_gradient() const { return kind == 0 && sp.caching == MFG_CACHING_TENSOR2 && sp.formulation == 1 && t2_storage == MFG_T2_GRADIENT; }
   DeviceBuffer flag;         // int error flag
