@@ -33,7 +33,7 @@ NUMBERING = "cellwise"
 # describes the kernel and the bench reports no traffic until it is re-measured.
 TRAFFIC_SOURCE = "profiles/vmult_q4_beam_fp64_pmc_traffic.md"
 KERNEL_SOURCE = os.path.join(ROOT, "matrix-free-dealii-precice_amd", "csrc", "cell_kernels.hpp")
-MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise", "gradient"): (2634991206, "377960fed9f7")}
+MEASURED_TRAFFIC = {((128, 32, 32), 4, "cellwise", "gradient"): (2633974579, "f2c744af6aeb")}
 
 
 def kernel_source_hash():
