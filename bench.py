@@ -125,9 +125,8 @@ def mgcg_bench(mfgpu, torch, cells, p, level_precision, numbering, x_true_seed=2
     precs = [mfgpu.FP64] + [level_precision] * (n_levels - 1)
     ops = [mfgpu.NeoHookOperator(m, MU, NU, RHO, ALPHA1, precision=pr) for m, pr in zip(meshes, precs)]
     fine64 = ops[-1] if level_precision == mfgpu.FP64 else mfgpu.NeoHookOperator(meshes[-1], MU, NU, RHO, ALPHA1)
-    for op in ops + [fine64]:
-        if op.precision == mfgpu.FP64:  # the FP64 operators take the bench's point storage
-            op.set_point_storage(mfgpu.T2_GRADIENT if storage == "gradient" else mfgpu.T2_FOLDED)
+    for op in ops + [fine64]:  # every level takes the bench's point storage
+        op.set_point_storage(mfgpu.T2_GRADIENT if storage == "gradient" else mfgpu.T2_FOLDED)
     trs = [mfgpu.Transfer(ops[l - 1], ops[l], mfgpu.box_children(meshes[l - 1], meshes[l])) for l in range(1, n_levels)]
     u = torch.from_numpy(meshes[-1].u0(amp=1e-3, noise=1e-5, seed=0)).to(dev)
     fine64.cache(u)
@@ -612,7 +611,7 @@ def main():
         r32, x32 = mgcg_bench(mfgpu, torch, args.cells, p, mfgpu.FP32, args.numbering, storage=args.point_storage)
         r32["err_vs_fp64_solution"] = float(torch.linalg.norm(x32.cpu() - x64) / torch.linalg.norm(x64))
         out["mgcg"] = {"workload": "MG-CG to ||r||/||b|| <= 1e-8, rhs = A x_true (seed 2), Chebyshev degree 5, "
-                                   "coarse FP64 CG to 1e-3", "fp64_point_storage": args.point_storage,
+                                   "coarse FP64 CG to 1e-3", "point_storage": args.point_storage,
                        "fp64": r64, "mixed_fp32_vcycle": r32,
                        "speedup_mixed_vs_fp64": round(r64["solve_s"] / r32["solve_s"], 3)}
     if not args.no_newton:

@@ -429,8 +429,9 @@ typedef struct mfg_solid_problem_t *mfg_solid_problem;
  * and level caches; the coarse level, outer CG, residual and Newton update
  * stay FP64).  material->alpha_1 is ignored (1/(beta dt^2) is used);
  * material->caching selects the operators' caching (initialize's `caching`
- * argument, mf_nh_operator.h:115-119).  The FP64 tensor2 operators of
- * formulation 1 on all-affine meshes keep the MFG_T2_GRADIENT point storage. */
+ * argument, mf_nh_operator.h:115-119).  The tensor2 operators of
+ * formulation 1 on all-affine meshes (FP64 and FP32) keep the
+ * MFG_T2_GRADIENT point storage. */
 int mfg_solid_problem_create(int32_t n_levels, const mfg_level_desc *levels, const int32_t *const *children,
                              const mfg_solid_params *material, const mfg_newmark_params *newmark,
                              const mfg_mg_params *mg, int32_t level_precision, mfg_solid_problem *out);

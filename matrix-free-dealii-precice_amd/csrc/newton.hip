@@ -272,11 +272,10 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
     op->sp = sp;
     op->stream = stream;
     op->setup_layout(d);
-    // the problem's operators never enter a batched launch, so its FP64
-    // levels on affine cells take the 10-word point storage (faster and
-    // smaller than the folded 16 words)
-    if (prec == MFG_FP64 && sp.formulation == 1 && sp.caching == MFG_CACHING_TENSOR2 && op->affine_ref)
-      op->t2_storage = MFG_T2_GRADIENT;
+    // the problem's operators never enter a batched launch, so its levels on
+    // affine cells take the 10-word point storage (faster and smaller than the
+    // folded 16 words, in FP64 and FP32)
+    if (sp.formulation == 1 && sp.caching == MFG_CACHING_TENSOR2 && op->affine_ref) op->t2_storage = MFG_T2_GRADIENT;
     op->setup_solid();
     return op;
   };
