@@ -477,6 +477,8 @@ def main():
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-step leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: 2 vmults, then 2 copies of a 1 GiB buffer (known bytes), no timing")
+    ap.add_argument("--pmc-probe-heat", action="store_true",
+                    help="counter probe of the heat leg's FP64 operator: 2 vmults, then 2 copies of a 1 GiB buffer")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
     ap.add_argument("--degree", type=int, default=BEAM["degree"])
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
@@ -505,6 +507,26 @@ def main():
     if args.only_serving:
         print(json.dumps({"serving": serving_bench(mfgpu)}), flush=True)
         return
+
+    def copies_of_known_size():
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+
+    if args.pmc_probe_heat:
+        # the heat leg's FP64 operator: 96^3 Q4 cube, c ~ U[0.5, 2] per point (seed 7)
+        hmesh = mfgpu.BoxMesh([0, 0, 0], [1, 1, 1], [96, 96, 96], 4, 1, 63, 1, numbering=args.numbering)
+        coef = np.random.default_rng(7).uniform(0.5, 2.0, hmesh.n_cells * 125)
+        hop = mfgpu.LaplaceOperator(hmesh, 0.1, coef, precision=mfgpu.FP64)
+        hsrc = mfgpu.fill_uniform(1, torch.empty(hmesh.n_dofs, dtype=torch.float64, device=dev))
+        hdst = torch.empty_like(hsrc)
+        for _ in range(2):
+            hop.vmult(hdst, hsrc)
+        copies_of_known_size()
+        log("pmc probe done")
+        return
     p = args.degree
     mesh = mfgpu.BoxMesh(BEAM["lo"], BEAM["hi"], args.cells, p, 3, 1, 7, numbering=args.numbering)
     n_dof, n_cell = mesh.n_dofs, mesh.n_cells
@@ -520,11 +542,7 @@ def main():
     if args.pmc_probe:
         for _ in range(2):
             op.vmult(dst, src)
-        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(2):
-            b.copy_(a)
-        torch.cuda.synchronize()
+        copies_of_known_size()
         log("pmc probe done")
         return
 
