@@ -357,8 +357,8 @@ def cachings_bench(mfgpu, torch, cells, p, numbering, reps=20):
     out, ref = {}, None
     for name in ("scalar_referential", "tensor2", "tensor2_gradient", "tensor4", "tensor4_ns"):
         op = mfgpu.NeoHookOperator(mesh, MU, NU, RHO, ALPHA1, caching=name.replace("_gradient", ""))
-        if name == "tensor2_gradient":
-            op.set_point_storage(mfgpu.T2_GRADIENT)
+        if name.startswith("tensor2"):
+            op.set_point_storage(mfgpu.T2_GRADIENT if name == "tensor2_gradient" else mfgpu.T2_FOLDED)
         op.cache(u0)
         dst = torch.empty_like(src)
         for _ in range(3):

@@ -225,8 +225,9 @@ int mfg_operator_qp_data(mfg_operator op, void *out, int32_t *n_fields);
  * reference-cell gradient of the linearisation point Grad_xi u[d][f] (9) and
  * s = mu - 2 lambda ln J, 10 words per point; vmult rebuilds Jinv_cur =
  * (J_ref + Grad_xi u)^{-1} and tau = mu F F^T - s I per point from the
- * per-cell J_ref.  Default FOLDED.  Changing the storage discards the cache:
- * call mfg_operator_cache (and compute_diagonal) again. */
+ * per-cell J_ref.  Default: GRADIENT on meshes whose cells are all affine,
+ * FOLDED otherwise.  Changing the storage discards the cache: call
+ * mfg_operator_cache (and compute_diagonal) again. */
 #define MFG_T2_FOLDED 0
 #define MFG_T2_GRADIENT 1
 int mfg_operator_set_point_storage(mfg_operator op, int32_t storage);

@@ -912,12 +912,16 @@ class Instance:
     counts while all stay even).  Solid: neo-Hookean tensor2 linearised at
     u0 = amp sin sin sin + noise (levels at the interpolated u), Dirichlet on
     x = lo.  Heat: M + dt c K with constant conductivity c, Dirichlet on all
-    faces."""
+    faces.  point_storage: the tensor2 point storage of the formulation-1
+    solid fine operator; folded by default, since the fused batched kernel
+    (one register allocation for all degrees) runs the 10-word storage slower
+    at several cells per task."""
 
     def __init__(self, kind, n_cells, degree, fine_precision=FP64, level_precision=FP32, lo=(0.0, 0.0, 0.0),
                  hi=(1.0, 1.0, 1.0), dt=0.1, conductivity=1.0, formulation=1, cheb_degree=5, smoothing_range=None,
                  eig_cg_iterations=None, coarse_eig_iterations=0, amp=1e-3, noise=1e-5, seed=0,
-                 material=(384615.38461538462, 0.3, 1000.0, 4e6), cell_schedule=None, numbering="cellwise"):
+                 material=(384615.38461538462, 0.3, 1000.0, 4e6), cell_schedule=None, numbering="cellwise",
+                 point_storage=T2_FOLDED):
         import numpy as np
         import torch
         self.kind, self.degree, self.conductivity = kind, degree, conductivity
@@ -937,6 +941,8 @@ class Instance:
                 return LaplaceOperator(m, dt, coef, precision=pr)
         self.levels = [make(m, level_precision) for m in self.meshes]
         self.fine = make(fine_mesh, fine_precision)
+        if solid and formulation == 1:
+            self.fine.set_point_storage(point_storage)
         if cell_schedule is not None:
             for op in self.levels + [self.fine]:
                 op.set_cell_schedule(cell_schedule)

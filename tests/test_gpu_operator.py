@@ -63,8 +63,8 @@ def test_solid_cache_matches_oracle(cuda, p):
     # the gradient storage: Grad_xi u and s = mu - 2 lambda ln J per point.
     # (J_ref + Grad_xi u)^-1 is the oracle's current-frame Jinv, and s is its
     # c2 / (2 c1) with c2 = 2 s / J, c1 = 1 / J.
-    assert go.point_storage() == mfgpu.T2_FOLDED
-    go.set_point_storage(mfgpu.T2_GRADIENT)
+    assert go.point_storage() == mfgpu.T2_GRADIENT  # the default on all-affine meshes
+    go.set_point_storage(mfgpu.T2_GRADIENT)  # no change: the cache stays
     go.cache(dev(u))
     assert go.n_fields() == 10
     g = go.qp_data().cpu().numpy()
@@ -494,7 +494,8 @@ def test_gradient_point_storage_errors(cuda):
     lo, hi, n, p = [0, 0, 0], [1.0, 0.6, 0.4], [3, 2, 2], 2
     gm = mfgpu.BoxMesh(lo, hi, n, p, 3, 1, 7)
     go = mfgpu.NeoHookOperator(gm, O.MU, O.NU, O.RHO, O.ALPHA1)
-    assert go.point_storage() == mfgpu.T2_FOLDED  # the default
+    assert go.point_storage() == mfgpu.T2_GRADIENT  # the default on all-affine meshes
+    go.set_point_storage(mfgpu.T2_FOLDED)
     u = torch.from_numpy(gm.u0(amp=1e-2, noise=1e-5)).cuda()
     go.cache(u)
     mem16 = go.memory_consumption()
