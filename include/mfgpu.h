@@ -553,7 +553,11 @@ typedef struct mfg_batch_tuning {
   double ms_gather;             /* the same with MFG_SCHEDULE_GATHER (batched vmult) */
 } mfg_batch_tuning;
 typedef struct mfg_batch_t *mfg_batch;
-/* All operators must be cached and share one precision. */
+/* All operators must be cached and share one precision.  The batch holds the
+ * operators' device pointers: an operator whose device buffers are
+ * reallocated later (mfg_operator_set_point_storage, say) makes the next
+ * application rebuild the batch's tables, and one that is not cached when the
+ * batch is applied is refused (MFG_ERR_STATE). */
 int mfg_batch_create(int32_t n, const mfg_operator *ops, mfg_batch *out);
 int mfg_batch_destroy(mfg_batch b);
 int mfg_batch_set_stream(mfg_batch b, void *hip_stream);
@@ -578,7 +582,9 @@ int mfg_batch_get_tuning(mfg_batch b, mfg_batch_tuning *result);
  * level by level (levels aligned at the coarse end) as one fixed program of
  * batched launches, captured once into a hipGraph and replayed (use_graph,
  * default on).  Every instance performs the operation sequence of its own
- * mfg_mg_vmult / mfg_cg_solve. */
+ * mfg_mg_vmult / mfg_cg_solve.  Its programs hold the operators' device
+ * pointers: after an operator's buffers are reallocated (a new point storage,
+ * say) the solver refuses to run (MFG_ERR_STATE) and must be created again. */
 typedef struct mfg_batch_solver_t *mfg_batch_solver;
 int mfg_batch_solver_create(int32_t n, const mfg_operator *fine, const mfg_mg *mgs, mfg_batch_solver *out);
 int mfg_batch_solver_destroy(mfg_batch_solver s);

@@ -793,6 +793,10 @@ struct BatchVmult {
   std::vector<const void *> last_src;
   std::vector<void *> last_dst;
 
+  // each operator's buffers_version and instance code when the tables were built
+  std::vector<uint64_t> versions;
+  std::vector<int> codes;
+
   BatchVmult(const std::vector<OperatorData *> &mo) : mops(mo) {
     const std::vector<const OperatorData *> ops(mo.begin(), mo.end());
     std::vector<int> ids(ops.size());
@@ -803,6 +807,35 @@ struct BatchVmult {
     MFG_HIP(hipMemset(error.ptr, 0, sizeof(int)));
     MFG_HIP(hipDeviceSynchronize());  // operator set-up on other streams is complete
     group->build(tu, stream);
+    record_versions();
+  }
+  void record_versions() {
+    versions.clear();
+    codes.clear();
+    for (const OperatorData *op : mops) {
+      versions.push_back(op->buffers_version);
+      codes.push_back(code_of(*op));
+    }
+  }
+  // The tables hold the operators' device pointers: an operator whose buffers
+  // were reallocated since they were built (a new point storage, say) makes
+  // the batch rebuild them; an operator without a cache is refused.
+  void validate() {
+    bool changed = false;
+    for (size_t i = 0; i < mops.size(); ++i) {
+      MFG_REQUIRE(mops[i]->ready, MFG_ERR_STATE, "batched operator used before cache()");
+      changed |= mops[i]->buffers_version != versions[i] || code_of(*mops[i]) != codes[i];
+    }
+    if (!changed) return;
+    MFG_HIP(hipStreamSynchronize(stream));
+    const std::vector<const OperatorData *> ops(mops.begin(), mops.end());
+    std::unique_ptr<ApplyGroup> g(new ApplyGroup(ops, group->ids));
+    g->build(tu, stream);
+    group = std::move(g);
+    gather.reset();
+    last_src.clear();
+    last_dst.clear();
+    record_versions();
   }
   void set_vectors(const void *const *src, void *const *dst) {
     const size_t n = group->ops.size();
@@ -830,10 +863,14 @@ struct BatchVmult {
     pool.run(stream, [&](int i, hipStream_t s) { group->ops[i]->apply(src[i], dst[i], add, s); });
   }
   void apply_gather(const void *const *src, void *const *dst, bool add) {
-    if (!gather) gather.reset(new BatchGather(mops, stream));
+    if (!gather) {
+      gather.reset(new BatchGather(mops, stream));
+      record_versions();  // building the operators' GATHER tables counts as a change of their buffers
+    }
     gather->apply(group->inst.ptr, src, dst, add, stream);
   }
   void apply(const void *const *src, void *const *dst, bool add) {
+    validate();
     if (tu.queue == MFG_SCHEDULE_STREAMS) {
       apply_streams(src, dst, add);
       return;
@@ -848,6 +885,7 @@ struct BatchVmult {
   // the fused schedules' tuning, then the STREAMS and GATHER schedules timed
   // against the best of them
   void tune(const void *const *src, void *const *dst, int reps) {
+    validate();
     set_vectors(src, dst);
     group->tune(tu, vecs.ptr, reps, stream, error.as<int>());
     hipEvent_t e0, e1;
@@ -964,6 +1002,24 @@ struct BatchSolver {
     upload_active();
     red.ensure(n);
     build();
+    versions = current_versions();
+  }
+  // buffers_version of every fine and level operator: the programs and tables
+  // hold their device pointers
+  std::vector<uint64_t> versions;
+  std::vector<uint64_t> current_versions() const {
+    std::vector<uint64_t> v;
+    for (int i = 0; i < n; ++i) {
+      v.push_back(fine[i]->buffers_version);
+      for (const MGLevel &L : mg[i]->levels) v.push_back(L.op->buffers_version);
+    }
+    return v;
+  }
+  void check_unchanged() const {
+    MFG_REQUIRE(current_versions() == versions, MFG_ERR_STATE,
+                "an operator's device buffers changed after the batched solver was created: create it again");
+    for (int i = 0; i < n; ++i)
+      MFG_REQUIRE(fine[i]->ready && mg[i]->ready, MFG_ERR_STATE, "batched solver operator or multigrid not set up");
   }
   ~BatchSolver() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -1274,6 +1330,7 @@ struct BatchSolver {
   }
 
   void tune(int reps) {
+    check_unchanged();
     MFG_HIP(hipDeviceSynchronize());
     // fine operators on the workspace vectors
     fine_group->tune(tu_fine, vec_hp.ptr, reps, stream, error.as<int>());
@@ -1325,6 +1382,7 @@ struct BatchSolver {
 
   // z = MG(r) for every instance
   void precondition(const void *const *src, void *const *dst) {
+    check_unchanged();
     if (concurrent) {
       std::string err;
       std::vector<int32_t> st(n, MFG_OK);
@@ -1437,6 +1495,7 @@ struct BatchSolver {
   // cg_solve with the multigrid preconditioner, all instances in lockstep
   void solve(const void *const *b, void *const *x, double tol_rel, int max_iter, mfg_solve_info *info,
              int32_t *status) {
+    check_unchanged();
     if (concurrent) {
       solve_concurrent(b, x, tol_rel, max_iter, info, status);
       return;

@@ -194,3 +194,54 @@ def test_batched_errors(cuda):
     t = torch.zeros(meshes[1].n_dofs, dtype=torch.float64, device="cuda")
     with pytest.raises(mfgpu.MfgError):  # aliasing source and destination
         b.vmult([s, t], [s, t])
+
+
+@pytest.mark.parametrize("schedule", [mfgpu.SCHEDULE_COLOUR, mfgpu.SCHEDULE_GATHER, mfgpu.SCHEDULE_STREAMS])
+def test_batched_vmult_follows_operator_changes(cuda, schedule):
+    """An operator changed after the batch was built (a new point storage
+    reallocates its cache): the batch refuses it until it is cached again,
+    then rebuilds its tables and matches the per-instance results again."""
+    print(f"batch after an operator change, schedule={schedule}", flush=True)
+    ops, meshes = make_ops(SPECS, mfgpu.FP64)
+    srcs = [mfgpu.fill_uniform(700 + i, torch.empty(m.n_dofs, dtype=torch.float64, device="cuda"))
+            for i, m in enumerate(meshes)]
+    b = mfgpu.Batch(ops)
+    b.set_schedule(schedule)
+    out = [torch.empty_like(s) for s in srcs]
+    b.vmult(out, srcs)
+    b.synchronize()
+    k = next(i for i, sp in enumerate(SPECS) if sp[0] == "solid" and sp[3] == 1 and sp[2] >= 3)
+    ops[k].set_point_storage(mfgpu.T2_GRADIENT)  # discards the cache
+    with pytest.raises(mfgpu.MfgError) as e:
+        b.vmult(out, srcs)
+    assert e.value.code == 5
+    m = meshes[k]
+    ops[k].cache(torch.from_numpy(m.u0(amp=1e-3, noise=1e-5, seed=k)).to("cuda", torch.float64))
+    ref = [torch.empty_like(s) for s in srcs]
+    for op, s, r in zip(ops, srcs, ref):
+        op.vmult(r, s)
+    for rep in range(2):
+        out = [torch.full_like(s, float("nan")) for s in srcs]
+        b.vmult(out, srcs)
+        b.synchronize()
+        for i, (o, r) in enumerate(zip(out, ref)):
+            assert torch.equal(o, r), (rep, i, SPECS[i])
+
+
+def test_batched_solver_refuses_changed_operators(cuda):
+    inst = instances(ISPECS[:3])
+    solver = mfgpu.BatchSolver([i.fine for i in inst], [i.mg for i in inst])
+    bs_ = [mfgpu.fill_uniform(900 + k, torch.empty(i.n_dofs, dtype=torch.float64, device="cuda"))
+           for k, i in enumerate(inst)]
+    for k, i in enumerate(inst):
+        bs_[k][torch.from_numpy(i.constrained).long().cuda()] = 0.0
+    xs = [torch.zeros_like(b) for b in bs_]
+    infos, status = solver.solve(bs_, xs, tol_rel=1e-6)
+    assert all(st == 0 for st in status)
+    k = next(j for j, sp in enumerate(ISPECS[:3]) if sp[0] == "solid")
+    fine = inst[k].fine
+    fine.set_point_storage(mfgpu.T2_GRADIENT if fine.point_storage() == mfgpu.T2_FOLDED else mfgpu.T2_FOLDED)
+    fine.cache(torch.from_numpy(inst[k].meshes[-1].u0(amp=1e-3, noise=1e-5, seed=k)).to("cuda", torch.float64))
+    with pytest.raises(mfgpu.MfgError) as e:
+        solver.solve(bs_, [torch.zeros_like(b) for b in bs_], tol_rel=1e-6)
+    assert e.value.code == 5
