@@ -414,7 +414,18 @@ typedef struct mfg_newmark_params {
    * (solve_linear_system, mf_elasticity.h:1998-2013; the reference's default
    * is on with 20 iterations, parameter_handling.h:183-184).  0 skips it. */
   int32_t cond_cg_iterations;
+  /* Preconditioner of the linear solves ("Preconditioner type",
+   * Solid::solve_linear_system, mf_elasticity.h:2026-2051): MFG_NEWTON_GMG
+   * (0, the default), MFG_NEWTON_JACOBI (PreconditionJacobi with
+   * jacobi_relaxation, the reference's "Preconditioner relaxation") or
+   * MFG_NEWTON_NONE.  As in update_matrix_free (:1320-1324) the multigrid
+   * levels are only set up for MFG_NEWTON_GMG. */
+  int32_t linear_preconditioner;
+  double jacobi_relaxation;
 } mfg_newmark_params;
+#define MFG_NEWTON_GMG 0
+#define MFG_NEWTON_JACOBI 1
+#define MFG_NEWTON_NONE 2
 typedef struct mfg_step_info {
   int32_t newton_iterations;
   int32_t linear_iterations;  /* summed over the Newton iterations */

@@ -95,6 +95,11 @@ void SolidProblem::set_stream(hipStream_t s) {
 void SolidProblem::update_matrix_free() {
   OperatorData &F = fine();
   F.cache(u.ptr, MFG_FP64);
+  if (nm.linear_preconditioner != MFG_NEWTON_GMG) {
+    // setup_operator_cache of the fine operator only (mf_elasticity.h:1320-1324)
+    if (nm.linear_preconditioner == MFG_NEWTON_JACOBI) F.compute_diagonal();
+    return;
+  }
   const int top = L - 1;
   if (fine64) {
     vec_convert(levels[top]->precision, u_lev[top].ptr, MFG_FP64, u.ptr, n(), stream);
@@ -152,7 +157,7 @@ void SolidProblem::step(mfg_step_info &info) {
       // the rest of the mixed-precision preconditioning work; the solution
       // does not depend on it.
       const CGVectors cv{ws.r.ptr, ws.z.ptr, ws.p.ptr, ws.h.ptr};
-      if (fine64) {
+      if (fine64 && nm.linear_preconditioner == MFG_NEWTON_GMG) {
         OperatorData &F32 = *levels.back();
         if (cond_b.bytes != (size_t)N * 4) {
           cond_b.allocate((size_t)N * 4);
@@ -168,7 +173,13 @@ void SolidProblem::step(mfg_step_info &info) {
     }
     vec_fill(MFG_FP64, du.ptr, N, 0.0, stream);
     mfg_solve_info si{};
-    cg_solve(F, MFG_PRECOND_MG, mg.get(), 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
+    if (nm.linear_preconditioner == MFG_NEWTON_JACOBI)
+      cg_solve(F, MFG_PRECOND_JACOBI, nullptr, nm.jacobi_relaxation, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws,
+               red);
+    else if (nm.linear_preconditioner == MFG_NEWTON_NONE)
+      cg_solve(F, MFG_PRECOND_NONE, nullptr, 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
+    else
+      cg_solve(F, MFG_PRECOND_MG, mg.get(), 1.0, R.ptr, du.ptr, nm.tol_lin, nm.max_lin, &si, ws, red);
     info.linear_iterations += si.iterations;
     red.dot_dev(MFG_FP64, du.ptr, du.ptr, N, kRedSet, kSlotUpd, stream);
     add2_kernel<double><<<grid, 256, 0, stream>>>(delta_u.as<double>(), u.as<double>(), du.as<double>(), N);
@@ -239,6 +250,10 @@ std::unique_ptr<SolidProblem> make_solid_problem(int n_levels, const mfg_level_d
   MFG_REQUIRE(n_levels == 1 || children, MFG_ERR_ARGUMENT, "children tables required");
   MFG_REQUIRE(level_precision == MFG_FP64 || level_precision == MFG_FP32, MFG_ERR_ARGUMENT, "bad precision");
   MFG_REQUIRE(newmark.dt > 0 && newmark.beta > 0, MFG_ERR_ARGUMENT, "bad Newmark parameters");
+  MFG_REQUIRE(newmark.linear_preconditioner >= MFG_NEWTON_GMG && newmark.linear_preconditioner <= MFG_NEWTON_NONE,
+              MFG_ERR_ARGUMENT, "Preconditioner type not implemented");
+  MFG_REQUIRE(newmark.linear_preconditioner != MFG_NEWTON_JACOBI || newmark.jacobi_relaxation > 0, MFG_ERR_ARGUMENT,
+              "the Jacobi preconditioner needs a positive relaxation");
   MFG_REQUIRE(material.mu > 0 && material.nu > -1 && material.nu < 0.5 && material.rho >= 0, MFG_ERR_ARGUMENT,
               "bad material");
   MFG_REQUIRE(material.inclusion_mu == 0 || (material.inclusion_mu > 0 && material.inclusion_nu > -1 &&

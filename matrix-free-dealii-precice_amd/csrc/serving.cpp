@@ -287,7 +287,9 @@ int mfg_serve(const mfg_session_spec *sessions, int32_t n_sessions, const mfg_re
     srv.prm = *params;
     for (int i = 0; i < n_sessions; ++i) {
       const mfg_newmark_params &nm = sessions[i].newmark;
-      MFG_REQUIRE(nm.dt > 0 && nm.beta > 0 && nm.gamma >= 0 && nm.max_newton >= 1 && nm.max_lin >= 1,
+      MFG_REQUIRE(nm.dt > 0 && nm.beta > 0 && nm.gamma >= 0 && nm.max_newton >= 1 && nm.max_lin >= 1 &&
+                      nm.linear_preconditioner >= MFG_NEWTON_GMG && nm.linear_preconditioner <= MFG_NEWTON_NONE &&
+                      (nm.linear_preconditioner != MFG_NEWTON_JACOBI || nm.jacobi_relaxation > 0),
                   MFG_ERR_ARGUMENT, "bad Newmark parameters");
       auto s = std::make_unique<Session>();
       s->spec = sessions[i];

@@ -600,7 +600,12 @@ class NewmarkParams(ctypes.Structure):
                 ("body_force", ctypes.c_double * 3), ("max_newton", ctypes.c_int32), ("min_newton", ctypes.c_int32),
                 ("tol_f", ctypes.c_double), ("tol_u", ctypes.c_double), ("tol_f_abs", ctypes.c_double),
                 ("tol_u_abs", ctypes.c_double), ("tol_lin", ctypes.c_double), ("max_lin", ctypes.c_int32),
-                ("cond_cg_iterations", ctypes.c_int32)]
+                ("cond_cg_iterations", ctypes.c_int32), ("linear_preconditioner", ctypes.c_int32),
+                ("jacobi_relaxation", ctypes.c_double)]
+
+
+# linear preconditioner of the Newton step ("Preconditioner type", parameter_handling.h:181-182)
+NEWTON_PRECONDITIONER = {"gmg": 0, "jacobi": 1, "none": 2}
 
 
 class StepInfo(ctypes.Structure):
@@ -716,16 +721,18 @@ class SolidProblem:
                  level_precision=FP64, max_newton=10, min_newton=3, tol_f=1e-8, tol_u=1e-6, tol_f_abs=0.0,
                  tol_u_abs=0.0, tol_lin=1e-6, max_lin=1000, cond_cg_iterations=20, cheb_degree=5, smoothing_range=20.0,
                  eig_cg_iterations=30, coarse_kind=0, coarse_range=1e-3, coarse_tolerance=1e-3, caching="tensor2",
-                 inclusion=None):
+                 inclusion=None, preconditioner="gmg", relaxation=0.65):
         """inclusion: (mu, nu, rho) of the material on cells whose material id is not 0 (every level's mesh
-        carries its own ids: set_material_ids, refined_material_ids)."""
+        carries its own ids: set_material_ids, refined_material_ids).  preconditioner: "gmg", "jacobi" (with
+        relaxation, the reference's default 0.65) or "none", the reference's "Preconditioner type"."""
         L = len(meshes)
         descs = (LevelDesc * L)(*[m.desc for m in meshes])
         self._children = [box_children(meshes[l - 1], meshes[l]) for l in range(1, L)]
         ch = (_c_i32p * max(L - 1, 1))(*([c.ctypes.data_as(_c_i32p) for c in self._children] or [None]))
         sp = _solid_params(mu, nu, rho, 0.0, formulation, CACHING[caching], inclusion)
         nm = NewmarkParams(beta, gamma, dt, (ctypes.c_double * 3)(*body), max_newton, min_newton, tol_f, tol_u,
-                           tol_f_abs, tol_u_abs, tol_lin, max_lin, cond_cg_iterations)
+                           tol_f_abs, tol_u_abs, tol_lin, max_lin, cond_cg_iterations,
+                           NEWTON_PRECONDITIONER[preconditioner], relaxation)
         mp = MGParams(cheb_degree, smoothing_range, eig_cg_iterations, coarse_kind, coarse_range, coarse_tolerance)
         h = _vp()
         check(lib().mfg_solid_problem_create(L, descs, ch, ctypes.byref(sp), ctypes.byref(nm), ctypes.byref(mp),
