@@ -480,6 +480,7 @@ def main():
     ap.add_argument("--pmc-probe-heat", action="store_true",
                     help="counter probe of the heat leg's FP64 operator: 2 vmults, then 2 copies of a 1 GiB buffer")
     ap.add_argument("--only-serving", action="store_true", help="run the serving leg alone")
+    ap.add_argument("--only-newton", action="store_true", help="run the Newton time-step leg alone")
     ap.add_argument("--degree", type=int, default=BEAM["degree"])
     ap.add_argument("--cells", type=int, nargs=3, default=BEAM["cells"])
     ap.add_argument("--numbering", default=NUMBERING, choices=["cellwise", "lexicographic"],
@@ -506,6 +507,9 @@ def main():
         return
     if args.only_serving:
         print(json.dumps({"serving": serving_bench(mfgpu)}), flush=True)
+        return
+    if args.only_newton:
+        print(json.dumps({"newton": newton_bench(mfgpu, torch, args.cells, args.degree, args.numbering)}), flush=True)
         return
 
     def copies_of_known_size():

@@ -67,6 +67,21 @@ void Reducer::cg_update_dev(int prec, void *x, void *r, const void *p, const voi
   epilogue(mode, kSlotRR, 0, 0, false, s);
 }
 
+void Reducer::jacobi_dot_dev(int prec, void *z, const void *invD, const void *r, double omega, int64_t n, int mode,
+                             hipStream_t s, bool record) {
+  if (prec == MFG_FP64)
+    jacobi_dot_kernel<double><<<kRedBlocks, kRedThreads, 0, s>>>((double *)z, (const double *)invD,
+                                                                 (const double *)r, omega, n, partials);
+  else
+    jacobi_dot_kernel<float><<<kRedBlocks, kRedThreads, 0, s>>>((float *)z, (const float *)invD, (const float *)r,
+                                                                (float)omega, n, partials);
+  epilogue(mode, 0, 0, 0, record, s);
+}
+
+void Reducer::repeat_epilogue(int mode, int slot, hipStream_t s, double tol, double rel, bool record) {
+  epilogue(mode, slot, tol, rel, record, s);
+}
+
 void Reducer::read(double *out, hipStream_t s) {
   MFG_HIP(hipMemcpyAsync(out, scal, kRedScalars * sizeof(double), hipMemcpyDeviceToHost, s));
   MFG_HIP(hipStreamSynchronize(s));
@@ -145,15 +160,6 @@ static void add_scaled_t(void *y, const void *a, double sc, const void *b, int64
 void vec_add_scaled(int prec, void *y, const void *a, double sc, const void *b, int64_t n, hipStream_t s) {
   if (prec == MFG_FP64) add_scaled_t<double>(y, a, sc, b, n, s);
   else add_scaled_t<float>(y, a, sc, b, n, s);
-  MFG_HIP(hipGetLastError());
-}
-template <typename T>
-static void jacobi_t(void *dst, const void *invD, const void *src, double omega, int64_t n, hipStream_t s) {
-  scaled_product_kernel<T><<<grid_for(n), 256, 0, s>>>((T *)dst, (const T *)invD, (const T *)src, (T)omega, n);
-}
-void vec_jacobi(int prec, void *dst, const void *invD, const void *src, double omega, int64_t n, hipStream_t s) {
-  if (prec == MFG_FP64) jacobi_t<double>(dst, invD, src, omega, n, s);
-  else jacobi_t<float>(dst, invD, src, omega, n, s);
   MFG_HIP(hipGetLastError());
 }
 
@@ -356,14 +362,13 @@ void MGLevel::estimate_eigenvalues(int n_it, Reducer &red) {
   }
   vec_fill(pr, sol.ptr, n, 0.0, s);
   vec_convert(pr, cg_r.ptr, pr, tmp.ptr, n, s);
-  vec_jacobi(pr, cg_z.ptr, A.inv_diag.ptr, cg_r.ptr, 1.0, n, s);
   // The iterations run without host round trips: the step lengths live in
   // the reducer's scalar block and history; the stop flag (breakdown, or
   // ||r|| <= 1e-15 ||r_0||) turns the remaining iterations into no-ops.  Long
   // runs (the coarse level's m() iterations) look at the flag every 16
   // iterations to skip the no-ops.
   red.ensure_hist(std::max(n_it, 1));
-  red.dot_dev(pr, cg_r.ptr, cg_z.ptr, n, kRedStart, 0, s);
+  red.jacobi_dot_dev(pr, cg_z.ptr, A.inv_diag.ptr, cg_r.ptr, 1.0, n, kRedStart, s);
   vec_convert(pr, cg_p.ptr, pr, cg_z.ptr, n, s);
   red.dot_dev(pr, cg_r.ptr, cg_r.ptr, n, kRedRes0, 0, s, 0.0, 1e-15);
   double sc[kRedScalars];
@@ -372,8 +377,7 @@ void MGLevel::estimate_eigenvalues(int n_it, Reducer &red) {
     red.dot_dev(pr, cg_p.ptr, cg_h.ptr, n, kRedAlpha, 0, s, 0, 0, true);
     red.cg_update_dev(pr, sol.ptr, cg_r.ptr, cg_p.ptr, cg_h.ptr, n, kRedRRStop, s);
     if (it + 1 == n_it) break;
-    vec_jacobi(pr, cg_z.ptr, A.inv_diag.ptr, cg_r.ptr, 1.0, n, s);
-    red.dot_dev(pr, cg_r.ptr, cg_z.ptr, n, kRedBeta, 0, s, 0, 0, true);
+    red.jacobi_dot_dev(pr, cg_z.ptr, A.inv_diag.ptr, cg_r.ptr, 1.0, n, kRedBeta, s, true);
     red.xpby_dev(pr, cg_p.ptr, cg_z.ptr, n, s);
     if (n_it > 32 && (it + 1) % 16 == 0) {
       red.read(sc, s);
@@ -642,10 +646,16 @@ void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, cons
   hipStream_t s = A.stream;
   if (precond == MFG_PRECOND_JACOBI) MFG_REQUIRE(A.has_diagonal, MFG_ERR_STATE, "compute_diagonal() not called");
   if (precond == MFG_PRECOND_MG) MFG_REQUIRE(mg && mg->ready, MFG_ERR_STATE, "multigrid not set up");
-  auto apply_precond = [&](void *z, const void *r) {
-    if (precond == MFG_PRECOND_NONE) vec_convert(pr, z, pr, r, n, s);
-    else if (precond == MFG_PRECOND_JACOBI) vec_jacobi(pr, z, A.inv_diag.ptr, r, omega, n, s);
-    else mg->vmult(z, r, pr);
+  // z = P r and (r, z) through the epilogue `mode`; the Jacobi step and the
+  // product in one pass
+  auto precond_dot = [&](int mode) {
+    if (precond == MFG_PRECOND_JACOBI) {
+      red.jacobi_dot_dev(pr, v.z, A.inv_diag.ptr, v.r, omega, n, mode, s);
+      return;
+    }
+    if (precond == MFG_PRECOND_NONE) vec_convert(pr, v.z, pr, v.r, n, s);
+    else mg->vmult(v.z, v.r, pr);
+    red.dot_dev(pr, v.r, v.z, n, mode, 0, s);
   };
   // The step lengths are computed on the device from reductions that stay in
   // the reducer's scalar block; the host reads the block once per iteration
@@ -666,8 +676,7 @@ void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, cons
   std::vector<double> alphas, betas;
   bool converged = res <= tol;
   if (!converged) {
-    apply_precond(v.z, v.r);
-    red.dot_dev(pr, v.r, v.z, n, kRedStart, 0, s);
+    precond_dot(kRedStart);
     vec_convert(pr, v.p, pr, v.z, n, s);
     for (int it = 1; it <= max_iter; ++it) {
       op_vmult(A, v.h, v.p);
@@ -683,8 +692,7 @@ void cg_solve(const OperatorData &A, int precond, MGData *mg, double omega, cons
         converged = true;
         break;
       }
-      apply_precond(v.z, v.r);
-      red.dot_dev(pr, v.r, v.z, n, kRedBeta, 0, s);
+      precond_dot(kRedBeta);
       red.xpby_dev(pr, v.p, v.z, n, s);
     }
     if (!converged && info->iterations == max_iter) {
@@ -720,7 +728,8 @@ double cg_condition_estimate(const OperatorData &A, const void *b, void *x, int 
     red.dot_dev(pr, v.p, v.h, n, kRedAlpha, 0, s, 0, 0, true);
     red.cg_update_dev(pr, x, v.r, v.p, v.h, n, kRedRRStop, s);
     if (it + 1 == n_it) break;
-    red.dot_dev(pr, v.r, v.r, n, kRedBeta, 0, s, 0, 0, true);
+    // (r, r) of the new iterate: the update's partial sums again
+    red.repeat_epilogue(kRedBeta, 0, s, 0, 0, true);
     red.xpby_dev(pr, v.p, v.r, n, s);
   }
   double sc[kRedScalars];

@@ -28,6 +28,12 @@ struct Reducer {
                double tol = 0, double rel = 0, bool record = false);
   // x += alpha p; r -= alpha h (alpha from the scalar block); r.r through `mode`
   void cg_update_dev(int prec, void *x, void *r, const void *p, const void *h, int64_t n, int mode, hipStream_t s);
+  // z = omega D^{-1} r, and r.z through `mode` (one pass over the vectors)
+  void jacobi_dot_dev(int prec, void *z, const void *invD, const void *r, double omega, int64_t n, int mode,
+                      hipStream_t s, bool record = false);
+  // the partial sums of the last reduction through another epilogue (a second
+  // dot product of the same vectors, without reading them again)
+  void repeat_epilogue(int mode, int slot, hipStream_t s, double tol = 0, double rel = 0, bool record = false);
   // p = z + beta p (beta from the scalar block)
   void xpby_dev(int prec, void *p, const void *z, int64_t n, hipStream_t s);
   // the scalar block, synchronising the stream
@@ -45,7 +51,6 @@ void vec_convert(int prec_dst, void *dst, int prec_src, const void *src, int64_t
 void vec_xpby(int prec, void *p, const void *d, double beta, int64_t n, hipStream_t s);
 void vec_residual(int prec, void *t, const void *b, int64_t n, hipStream_t s);
 void vec_add_scaled(int prec, void *y, const void *a, double sc, const void *b, int64_t n, hipStream_t s);
-void vec_jacobi(int prec, void *dst, const void *invD, const void *src, double omega, int64_t n, hipStream_t s);
 
 struct TransferData {
   OperatorData *coarse = nullptr, *fine = nullptr;

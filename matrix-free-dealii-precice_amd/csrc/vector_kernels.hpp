@@ -226,6 +226,22 @@ __global__ void __launch_bounds__(kRedThreads) cg_update_dev_kernel(T *x, T *r, 
   block_reduce_store<T>(v, partials);
 }
 
+// z = omega D^{-1} r ; partial r.z: the Jacobi step and the (r, z) product of
+// a preconditioned CG in one pass, with the values of scaled_product_kernel
+// and the partial sums of dot_partial_kernel (same expressions, same grid)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) jacobi_dot_kernel(T *z, const T *invD, const T *r, T omega, int64_t n,
+                                                                  double *partials) {
+  double v = 0;
+  MFG_GRID_STRIDE(i, n) {
+    const T ri = r[i];
+    const T zi = omega * invD[i] * ri;
+    z[i] = zi;
+    v += (double)ri * (double)zi;
+  }
+  block_reduce_store<T>(v, partials);
+}
+
 // p = d + beta p with beta = scal[kSlotBeta]
 template <typename T>
 __global__ void xpby_dev_kernel(T *p, const T *d, const double *scal, int64_t n) {
