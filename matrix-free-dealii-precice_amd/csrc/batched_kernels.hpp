@@ -365,7 +365,14 @@ __global__ void __launch_bounds__(kRedThreads)
   double v = 0;
   if (!active || active[sg.inst]) {
     const T *x = (const T *)sg.a, *z = (const T *)sg.b;
-    MFG_GRID_STRIDE(i, sg.n) v += (double)x[i] * (double)z[i];
+    T xv[kRedBatch], zv[kRedBatch];
+    red_batches(
+        sg.n,
+        [&](int64_t i, int u) {
+          xv[u] = x[i];
+          zv[u] = z[i];
+        },
+        [&](int64_t, int u) { v += (double)xv[u] * (double)zv[u]; });
   }
   block_reduce_store<T>(v, out);
 }
@@ -381,12 +388,21 @@ __global__ void __launch_bounds__(kRedThreads)
     T *x = (T *)sg.y, *r = (T *)sg.a;
     const T *p = (const T *)sg.b, *h = (const T *)sg.c;
     const double alpha = dyn[sg.inst];
-    MFG_GRID_STRIDE(i, sg.n) {
-      x[i] = (T)((double)x[i] + alpha * (double)p[i]);
-      const T ri = (T)((double)r[i] - alpha * (double)h[i]);
-      r[i] = ri;
-      v += (double)ri * (double)ri;
-    }
+    T rv[kRedBatch], xv[kRedBatch], pv[kRedBatch], hv[kRedBatch];
+    red_batches(
+        sg.n,
+        [&](int64_t i, int u) {
+          rv[u] = r[i];
+          xv[u] = x[i];
+          pv[u] = p[i];
+          hv[u] = h[i];
+        },
+        [&](int64_t i, int u) {
+          x[i] = (T)((double)xv[u] + alpha * (double)pv[u]);
+          const T ri = (T)((double)rv[u] - alpha * (double)hv[u]);
+          r[i] = ri;
+          v += (double)ri * (double)ri;
+        });
   }
   block_reduce_store<T>(v, out);
 }

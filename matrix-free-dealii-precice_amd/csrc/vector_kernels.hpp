@@ -70,10 +70,40 @@ __device__ inline void block_reduce_store(double v, double *partials) {
   if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
 }
 
+// The reduction kernels take kRedBatch grid-stride iterations at a time while
+// a whole batch is in range: the loads of the batch issue before the first
+// use (more bytes in flight per wave); the rest go one by one.  Each thread
+// still visits its indices, and sums, in increasing order, so the partial
+// sums are those of the plain grid-stride loop.  load(i, u) fills slot u,
+// use(i, u) consumes it.
+constexpr int kRedBatch = 4;
+template <typename Load, typename Use>
+__device__ inline void red_batches(int64_t n, Load load, Use use) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (kRedBatch - 1) * stride < n; i += kRedBatch * stride) {
+#pragma unroll
+    for (int u = 0; u < kRedBatch; ++u) load(i + u * stride, u);
+#pragma unroll
+    for (int u = 0; u < kRedBatch; ++u) use(i + u * stride, u);
+  }
+  for (; i < n; i += stride) {
+    load(i, 0);
+    use(i, 0);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads) dot_partial_kernel(const T *a, const T *b, int64_t n, double *partials) {
   double v = 0;
-  MFG_GRID_STRIDE(i, n) v += (double)a[i] * (double)b[i];
+  T av[kRedBatch], bv[kRedBatch];
+  red_batches(
+      n,
+      [&](int64_t i, int u) {
+        av[u] = a[i];
+        bv[u] = b[i];
+      },
+      [&](int64_t, int u) { v += (double)av[u] * (double)bv[u]; });
   block_reduce_store<T>(v, partials);
 }
 
@@ -214,15 +244,26 @@ __global__ void __launch_bounds__(kRedThreads) cg_update_dev_kernel(T *x, T *r, 
                                                                      double *partials) {
   const double alpha = scal[kSlotAlpha];
   double v = 0;
-  MFG_GRID_STRIDE(i, n) {
-    T ri = r[i];
-    if (alpha != 0) {
-      x[i] = (T)((double)x[i] + alpha * (double)p[i]);
-      ri = (T)((double)ri - alpha * (double)h[i]);
-      r[i] = ri;
-    }
-    v += (double)ri * (double)ri;
-  }
+  T rv[kRedBatch], xv[kRedBatch], pv[kRedBatch], hv[kRedBatch];
+  if (alpha != 0)
+    red_batches(
+        n,
+        [&](int64_t i, int u) {
+          rv[u] = r[i];
+          xv[u] = x[i];
+          pv[u] = p[i];
+          hv[u] = h[i];
+        },
+        [&](int64_t i, int u) {
+          x[i] = (T)((double)xv[u] + alpha * (double)pv[u]);
+          const T ri = (T)((double)rv[u] - alpha * (double)hv[u]);
+          r[i] = ri;
+          v += (double)ri * (double)ri;
+        });
+  else
+    red_batches(
+        n, [&](int64_t i, int u) { rv[u] = r[i]; },
+        [&](int64_t, int u) { v += (double)rv[u] * (double)rv[u]; });
   block_reduce_store<T>(v, partials);
 }
 
@@ -233,12 +274,18 @@ template <typename T>
 __global__ void __launch_bounds__(kRedThreads) jacobi_dot_kernel(T *z, const T *invD, const T *r, T omega, int64_t n,
                                                                   double *partials) {
   double v = 0;
-  MFG_GRID_STRIDE(i, n) {
-    const T ri = r[i];
-    const T zi = omega * invD[i] * ri;
-    z[i] = zi;
-    v += (double)ri * (double)zi;
-  }
+  T rv[kRedBatch], dv[kRedBatch];
+  red_batches(
+      n,
+      [&](int64_t i, int u) {
+        rv[u] = r[i];
+        dv[u] = invD[i];
+      },
+      [&](int64_t i, int u) {
+        const T zi = omega * dv[u] * rv[u];
+        z[i] = zi;
+        v += (double)rv[u] * (double)zi;
+      });
   block_reduce_store<T>(v, partials);
 }
 
