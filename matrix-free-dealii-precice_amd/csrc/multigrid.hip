@@ -487,27 +487,36 @@ void MGData::coarse_solve() {
   hipStream_t s = A.stream;
   vec_fill(pr, L.sol.ptr, n, 0.0, s);
   vec_convert(pr, L.cg_r.ptr, pr, L.def.ptr, n, s);
-  // step lengths stay on the device; one read-back per iteration (||r||)
-  double sc[kRedScalars];
-  red.dot_dev(pr, L.cg_r.ptr, L.cg_r.ptr, n, kRedSet, kSlotRR, s);
-  red.read(sc, s);
-  const double res0 = std::sqrt(sc[kSlotRR]);
-  const double tol = std::max(1e-10, prm.coarse_tolerance * res0);
-  if (res0 <= tol) return;
+  // The step lengths and the stop test (||r|| <= max(1e-10, coarse_tolerance
+  // ||r_0||), or a breakdown) stay on the device: once the stop flag is set,
+  // the remaining iterations change neither sol nor r.  The host reads the
+  // scalar block after a run of iterations as long as the previous coarse
+  // solve took, then after runs of 4, so a coarse solve usually costs one
+  // read-back instead of one per iteration.
   L.chebyshev(L.cg_z.ptr, L.cg_r.ptr, true);
   red.dot_dev(pr, L.cg_r.ptr, L.cg_z.ptr, n, kRedStart, 0, s);
   vec_convert(pr, L.cg_p.ptr, pr, L.cg_z.ptr, n, s);
+  red.dot_dev(pr, L.cg_r.ptr, L.cg_r.ptr, n, kRedRes0, 0, s, 1e-10, prm.coarse_tolerance);
   const int maxit = std::max<int>((int)n, 100);
-  for (int it = 0; it < maxit; ++it) {
-    op_vmult(A, L.cg_h.ptr, L.cg_p.ptr);
-    red.dot_dev(pr, L.cg_p.ptr, L.cg_h.ptr, n, kRedAlpha, 0, s);
-    red.cg_update_dev(pr, L.sol.ptr, L.cg_r.ptr, L.cg_p.ptr, L.cg_h.ptr, n, kRedSet, s);
+  double sc[kRedScalars];
+  int it = 0, run = std::max(1, coarse_iterations);
+  for (;;) {
+    for (const int end = std::min(maxit, it + run); it < end; ++it) {
+      if (it > 0) {
+        L.chebyshev(L.cg_z.ptr, L.cg_r.ptr, true);
+        red.dot_dev(pr, L.cg_r.ptr, L.cg_z.ptr, n, kRedBeta, 0, s);
+        red.xpby_dev(pr, L.cg_p.ptr, L.cg_z.ptr, n, s);
+      }
+      op_vmult(A, L.cg_h.ptr, L.cg_p.ptr);
+      red.dot_dev(pr, L.cg_p.ptr, L.cg_h.ptr, n, kRedAlpha, 0, s, 0, 0, true);
+      red.cg_update_dev(pr, L.sol.ptr, L.cg_r.ptr, L.cg_p.ptr, L.cg_h.ptr, n, kRedRRStop, s);
+    }
     red.read(sc, s);
-    if (sc[kSlotBreakdown] != 0 || std::sqrt(sc[kSlotRR]) <= tol) return;
-    L.chebyshev(L.cg_z.ptr, L.cg_r.ptr, true);
-    red.dot_dev(pr, L.cg_r.ptr, L.cg_z.ptr, n, kRedBeta, 0, s);
-    red.xpby_dev(pr, L.cg_p.ptr, L.cg_z.ptr, n, s);
+    if (sc[kSlotStop] != 0 || it >= maxit) break;
+    run = 4;
   }
+  // active iterations (counted while the step-length history has room)
+  coarse_iterations = (int)sc[kSlotNAlpha];
 }
 
 void MGData::down_sweep() {

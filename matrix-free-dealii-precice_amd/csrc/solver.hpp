@@ -87,6 +87,9 @@ struct MGData {
   // the preconditioner non-linear (outer CG stagnates), so the Chebyshev
   // solver mode is used there whatever prm.coarse_kind says.
   int coarse_kind() const { return levels.size() == 1 ? 0 : prm.coarse_kind; }
+  // active iterations of the last coarse CG: the length of the next one's
+  // first run between read-backs
+  int coarse_iterations = 0;
   void setup();
   void vmult(void *dst, const void *src, int vec_prec);
   void v_cycle();

@@ -148,7 +148,7 @@ enum RedSlot : int {
 // Epilogue of the final reduction pass (v = the reduced sum):
 //   kRedSet     scal[slot] = v
 //   kRedStart   (r, z) of the first iterate: resets the flags and the history
-//   kRedRes0    ||r_0|| = sqrt(v) and the stop threshold (tol, rel); stop at once if ||r_0|| <= tol
+//   kRedRes0    ||r_0|| = sqrt(v) and the stop threshold (tol, rel); stop at once if ||r_0|| <= max(tol, rel ||r_0||)
 //   kRedAlpha   (p, A p): alpha = gh / v and record it, or stop on breakdown
 //   kRedRRStop  (r, r) after the update; stop when below the threshold
 //   kRedBeta    (r, z) of the new iterate: beta = v / gh and record it
@@ -175,7 +175,7 @@ __device__ inline void red_epilogue(double v, double *scal, int mode, int slot, 
       scal[kSlotRes0] = r0;
       scal[kSlotTol] = tol;
       scal[kSlotRel] = rel;
-      if (!(r0 > tol)) scal[kSlotStop] = 1;
+      if (!(r0 > fmax(tol, rel * r0))) scal[kSlotStop] = 1;
       break;
     }
     case kRedAlpha:
