@@ -229,3 +229,31 @@ def test_graphed_vcycle_equals_eager(cuda, coarse_kind, precisions):
     i1 = mfgpu.cg_solve(fine, b, x1, mfgpu.PRECOND_MG, eager, tol_rel=1e-8)
     i2 = mfgpu.cg_solve(fine, b, x2, mfgpu.PRECOND_MG, graphed, tol_rel=1e-8)
     assert i1.iterations == i2.iterations and torch.equal(x1, x2)
+
+
+def test_coarse_cg_results_independent_of_history(cuda):
+    """The V-cycle's coarse CG stops on the device and the host reads its
+    state after runs of iterations sized by the previous coarse solve: the
+    results must not depend on that history.  The same input gives bitwise the
+    same output after another input (another coarse residual history) and on a
+    fresh hierarchy whose first coarse solve reads back after one iteration."""
+    print("coarse CG history independence", flush=True)
+    gms, oms, gops, oops, trs = hierarchy("solid", [0, 0, 0], [2.0, 1.0, 1.0], [2, 1, 1], 3, 3)
+    n = gms[-1].n_dofs
+    x = mfgpu.fill_uniform(40, torch.empty(n, dtype=torch.float64, device="cuda"))
+    y = torch.zeros_like(x)
+    y[n // 3] = 1.0  # a point load
+    mg = mfgpu.Multigrid(gops, trs, cheb_degree=5, coarse_kind=1, coarse_tolerance=1e-6)
+    mg.setup()
+    outs = []
+    for src in (x, y, x, y, x):
+        out = torch.full_like(src, float("nan"))
+        mg.vmult(out, src)
+        outs.append(out)
+    fresh = mfgpu.Multigrid(gops, trs, cheb_degree=5, coarse_kind=1, coarse_tolerance=1e-6)
+    fresh.setup()
+    first = torch.full_like(x, float("nan"))
+    fresh.vmult(first, x)
+    assert torch.isfinite(outs[0]).all() and torch.isfinite(outs[1]).all()
+    assert torch.equal(outs[0], outs[2]) and torch.equal(outs[0], outs[4]) and torch.equal(outs[0], first)
+    assert torch.equal(outs[1], outs[3])
